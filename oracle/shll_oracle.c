@@ -1,0 +1,514 @@
+/*
+ * shll_oracle.c -- CPU restatement of the SHLL time-step loop (see shll_oracle.h).
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP library, never the product.
+ *
+ * Build: gcc -O3 -ffp-contract=off -fopenmp (no -march, no -ffast-math), so every
+ * float/double operation is one IEEE-754 operation in program order, like the
+ * reference's x86-64 gcc -O3 build (base-c/makefile:2-4).
+ *
+ * How each function is pinned (tests/test_oracle_*.py):
+ *   REF32 u_from_p / p_from_u / flux / gather / update / minmod, all four schemes:
+ *       bitwise against golden vectors written by the compiled reference
+ *       (oracle/_ref, tests/golden): base_shll.c full Sod run (N=256, 410 steps),
+ *       base_shll_2d.c implosion (256^2, 1/10/205 steps), 2nd_order_base_shll.c four
+ *       shocks (256^2, 1/10 steps), plus injected random / circle / y-uniform Sod
+ *       initial data.  The derived 1D second-order scheme is pinned by the 2D
+ *       second-order reference run on y-uniform Sod data (every row equal).
+ *   F64: same source text as REF32 (double instantiation); additionally pinned by
+ *       invariants: rest state is a fixed point, F+ + F- = F, M=0 closed form,
+ *       mirror symmetry of a mirrored Sod run, mass/energy conservation with
+ *       reflective walls, momentum change = wall impulse, 2D run on y-uniform data
+ *       equals the 1D run, L1 convergence to the exact Riemann solution.
+ *   C32: no reference exists (it is this project's compensated scheme); pinned by
+ *       the same invariants and by its distance to F64 (L1(rho) <= 1e-6).
+ *   Initial conditions: bitwise against the reference's own initial arrays
+ *       (golden step-0 dumps) for Sod, implosion and four shocks; the circle by a
+ *       brute-force count of cells inside the radius.
+ *   orc_steps_to_time: the published step counts (README.md:89-99, :201-204, :247-248).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include "shll_oracle.h"
+
+static double orc_now(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+/* ---- two instantiations of the reference arithmetic ---- */
+#define RT float
+#define SFX(x) x##_r32
+#include "oracle_scheme.inc"
+#undef RT
+#undef SFX
+
+#define RT double
+#define SFX(x) x##_f64
+#include "oracle_scheme.inc"
+#undef RT
+#undef SFX
+
+int orc_nvar(const orc_problem *pb) { return pb->dim == 1 ? 3 : 4; }
+
+static int orc_check(const orc_problem *pb)
+{
+    if (!pb) return -1;
+    if (pb->dim != 1 && pb->dim != 2) return -1;
+    if (pb->order != 1 && pb->order != 2) return -1;
+    if (pb->bc != ORC_BC_REFLECT && pb->bc != ORC_BC_OUTFLOW) return -1;
+    if (pb->nx < 2) return -1;
+    if (pb->dim == 1 && pb->ny != 1) return -1;
+    if (pb->dim == 2 && pb->ny < 2) return -1;
+    if (!(pb->lx > 0) || (pb->dim == 2 && !(pb->ly > 0))) return -1;
+    if ((double)pb->nx * (double)pb->ny > 2.0e9) return -1;
+    return 0;
+}
+
+/* base_shll.c:200-218: float time = 0; while (time < TOTAL_TIME) { ...; time += DT; NO_STEPS++; } */
+int64_t orc_steps_to_time(const orc_problem *pb, float t_end)
+{
+    cst_r32 k;
+    float time = 0.0f;
+    int64_t n = 0;
+    make_cst_r32(&k, pb->nx, pb->ny, pb->lx, pb->dim == 2 ? pb->ly : 1.0);
+    while (time < t_end) {
+        time += k.dt;
+        n++;
+    }
+    return n;
+}
+
+/* ------------------------------------------------------------------------- */
+/* FP32-compensated mode.                                                    */
+/* Constants are the FP64 oracle's, rounded once to float.  Fluxes are float  */
+/* functions of the hi word; the increment is added to (hi, lo) with TwoSum   */
+/* (Knuth, TAOCP vol. 2, 4.2.2), so hi+lo carries ~48 significant bits.       */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+    int nx, ny, n, nv, dim, order, bc;
+    float R, gR, rcv, lamx, lamy, hlamx, hlamy;
+    double cv64, gR64;
+    float *hi[4], *lo[4], *fp[4], *fm[4], *hp[4], *hm[4];
+    void *block;
+} c32grid;
+
+static void two_sum(float a, float b, float *s, float *e)
+{
+    float ss = a + b;
+    float bb = ss - a;
+    *e = (a - (ss - bb)) + (b - bb);
+    *s = ss;
+}
+
+static float mm_c32(float l, float r)
+{
+    if (l * r < 0.0f) return 0.0f;
+    return fabsf(l) < fabsf(r) ? l : r;
+}
+
+static int c32_alloc(c32grid *g, const orc_problem *pb)
+{
+    cst_f64 k;
+    int v;
+    size_t n = (size_t)pb->nx * (size_t)pb->ny;
+    float *base;
+    memset(g, 0, sizeof(*g));
+    g->nx = pb->nx; g->ny = pb->ny; g->n = (int)n; g->nv = orc_nvar(pb);
+    g->dim = pb->dim; g->order = pb->order; g->bc = pb->bc;
+    make_cst_f64(&k, pb->nx, pb->ny, pb->lx, pb->dim == 2 ? pb->ly : 1.0);
+    g->R = (float)k.R;
+    g->gR = (float)k.gR;
+    g->rcv = (float)(1.0 / k.cv);
+    g->lamx = (float)k.dtdx;
+    g->lamy = (float)k.dtdy;
+    g->hlamx = 0.5f * g->lamx;
+    g->hlamy = 0.5f * g->lamy;
+    g->cv64 = k.cv;
+    g->gR64 = k.gR;
+    g->block = malloc(n * sizeof(float) * (size_t)(g->nv * (pb->dim == 1 ? 4 : 6)) + 64);
+    if (!g->block) return -1;
+    base = (float *)g->block;
+    for (v = 0; v < g->nv; v++) {
+        g->hi[v] = base; base += n;
+        g->lo[v] = base; base += n;
+        g->fp[v] = base; base += n;
+        g->fm[v] = base; base += n;
+        if (pb->dim == 2) { g->hp[v] = base; base += n; g->hm[v] = base; base += n; }
+    }
+    return 0;
+}
+
+/* FP64 primitive -> conserved with the scheme's kinetic-energy association. */
+static void p2u_f64(int dim, int ke, double cv, const double *p, double *u)
+{
+    if (dim == 1) {
+        u[0] = p[0];
+        u[1] = p[0] * p[1];
+        u[2] = ke == 0 ? p[0] * (p[2] * cv + (0.5 * p[1]) * p[1])
+                       : p[0] * (p[2] * cv + 0.5 * (p[1] * p[1]));
+    } else {
+        u[0] = p[0];
+        u[1] = p[0] * p[1];
+        u[2] = p[0] * p[2];
+        u[3] = p[0] * (p[3] * cv + 0.5 * (p[1] * p[1] + p[2] * p[2]));
+    }
+}
+
+static void u2p_f64(int dim, int ke, double cv, const double *u, double *p)
+{
+    double r = u[0];
+    double vx = u[1] / r;
+    if (dim == 1) {
+        p[0] = r; p[1] = vx;
+        p[2] = ke == 0 ? ((u[2] / r) - (0.5 * vx) * vx) / cv : ((u[2] / r) - 0.5 * (vx * vx)) / cv;
+    } else {
+        double vy = u[2] / r;
+        p[0] = r; p[1] = vx; p[2] = vy;
+        p[3] = ((u[3] / r) - 0.5 * (vx * vx + vy * vy)) / cv;
+    }
+}
+
+static void c32_init(c32grid *g, const double *prim)
+{
+    int c, v, ke = (g->dim == 1 && g->order == 1) ? 0 : 1;
+    for (c = 0; c < g->n; c++) {
+        double p[4], u[4];
+        for (v = 0; v < g->nv; v++) p[v] = prim[(size_t)v * g->n + c];
+        p2u_f64(g->dim, ke, g->cv64, p, u);
+        for (v = 0; v < g->nv; v++) {
+            float hi = (float)u[v];
+            g->hi[v][c] = hi;
+            g->lo[v][c] = (float)(u[v] - (double)hi);
+        }
+    }
+}
+
+/* Per-cell float flux of the compensated mode. */
+static void c32_cell_flux(c32grid *g, int c)
+{
+    int v;
+    if (g->dim == 1) {
+        float r = g->hi[0][c], m = g->hi[1][c], E = g->hi[2][c];
+        float inv = 1.0f / r;
+        float vx = m * inv, e = E * inv;
+        float T = (e - 0.5f * (vx * vx)) * g->rcv;
+        float a = sqrtf(g->gR * T);
+        float M = vx / a;
+        float z1 = 0.5f * (M + 1.0f), z2 = (0.5f * a) * (1.0f - M * M), z3 = 0.5f * (M - 1.0f);
+        float P = (r * g->R) * T;
+        float f[3], U[3];
+        f[0] = m; f[1] = m * vx + P; f[2] = vx * (E + P);
+        U[0] = r; U[1] = m; U[2] = E;
+        for (v = 0; v < 3; v++) {
+            g->fp[v][c] = f[v] * z1 + U[v] * z2;
+            g->fm[v][c] = -f[v] * z3 - U[v] * z2;
+        }
+    } else {
+        float r = g->hi[0][c], mx = g->hi[1][c], my = g->hi[2][c], E = g->hi[3][c];
+        float inv = 1.0f / r;
+        float vx = mx * inv, vy = my * inv, e = E * inv;
+        float T = (e - 0.5f * (vx * vx + vy * vy)) * g->rcv;
+        float a = sqrtf(g->gR * T);
+        float P = (r * g->R) * T;
+        float U[4], f[4], z1, z2, z3, M;
+        U[0] = r; U[1] = mx; U[2] = my; U[3] = E;
+        M = vx / a;
+        z1 = 0.5f * (M + 1.0f); z2 = (0.5f * a) * (1.0f - M * M); z3 = 0.5f * (M - 1.0f);
+        f[0] = mx; f[1] = mx * vx + P; f[2] = mx * vy; f[3] = vx * (E + P);
+        for (v = 0; v < 4; v++) {
+            g->fp[v][c] = f[v] * z1 + U[v] * z2;
+            g->fm[v][c] = -f[v] * z3 - U[v] * z2;
+        }
+        M = vy / a;
+        z1 = 0.5f * (M + 1.0f); z2 = (0.5f * a) * (1.0f - M * M); z3 = 0.5f * (M - 1.0f);
+        f[0] = my; f[1] = my * vx; f[2] = my * vy + P; f[3] = vy * (E + P);
+        for (v = 0; v < 4; v++) {
+            g->hp[v][c] = f[v] * z1 + U[v] * z2;
+            g->hm[v][c] = -f[v] * z3 - U[v] * z2;
+        }
+    }
+}
+
+static const float sx1_c32[3] = {-1.0f, 1.0f, -1.0f};
+static const float sx2_c32[4] = {-1.0f, 1.0f, -1.0f, -1.0f};
+static const float sy2_c32[4] = {-1.0f, -1.0f, 1.0f, -1.0f};
+
+/* slope of the split flux q at line position s (0 on the two boundary cells) */
+static float c32_slope(const float *q, int c, int s, int stride, int len)
+{
+    if (s <= 0 || s >= len - 1) return 0.0f;
+    return mm_c32(q[c] - q[c - stride], q[c + stride] - q[c]);
+}
+
+/* First- and second-order increments along one direction, same gather rules as
+ * the reference modes (gather_cell / slope_cell of oracle_scheme.inc). */
+static void c32_dir(const c32grid *g, float **qp, float **qm, const float *sgn, int v, int c, int s,
+                    int stride, int len, float lam, float hlam, float *d1, float *d2)
+{
+    float Lv, Rv;
+    if (s == 0) {
+        Lv = g->bc == ORC_BC_REFLECT ? sgn[v] * qm[v][c] : qp[v][c];
+        Rv = qm[v][c + stride];
+    } else if (s == len - 1) {
+        Lv = qp[v][c - stride];
+        Rv = g->bc == ORC_BC_REFLECT ? sgn[v] * qp[v][c] : qm[v][c];
+    } else {
+        Lv = qp[v][c - stride];
+        Rv = qm[v][c + stride];
+    }
+    *d1 = lam * (((qp[v][c] - qm[v][c]) + Rv) - Lv);
+    *d2 = 0.0f;
+    if (g->order == 2) {
+        float dp = c32_slope(qp[v], c, s, stride, len);
+        float dm = c32_slope(qm[v], c, s, stride, len);
+        float Ldf = s == 0 ? 0.0f : c32_slope(qp[v], c - stride, s - 1, stride, len);
+        float Rdf = s == len - 1 ? 0.0f : c32_slope(qm[v], c + stride, s + 1, stride, len);
+        *d2 = hlam * (((dp + dm) - Rdf) - Ldf);
+    }
+}
+
+static void c32_step(c32grid *g)
+{
+    int c, v;
+    for (c = 0; c < g->n; c++) c32_cell_flux(g, c);
+    for (c = 0; c < g->n; c++) {
+        int i = g->dim == 1 ? c : c / g->ny;
+        int j = g->dim == 1 ? 0 : c % g->ny;
+        for (v = 0; v < g->nv; v++) {
+            float dx1, dx2, dy1 = 0.0f, dy2 = 0.0f, y;
+            c32_dir(g, g->fp, g->fm, g->dim == 1 ? sx1_c32 : sx2_c32, v, c, i,
+                    g->dim == 1 ? 1 : g->ny, g->nx, g->lamx, g->hlamx, &dx1, &dx2);
+            if (g->dim == 2)
+                c32_dir(g, g->hp, g->hm, sy2_c32, v, c, j, 1, g->ny, g->lamy, g->hlamy, &dy1, &dy2);
+            y = g->lo[v][c] - dx1;
+            if (g->order == 2) y = y - dx2;
+            if (g->dim == 2) {
+                y = y - dy1;
+                if (g->order == 2) y = y - dy2;
+            }
+            /* the new fluxes of this step were all computed from the old hi above,
+             * so the state can be overwritten in place */
+            two_sum(g->hi[v][c], y, &g->hi[v][c], &g->lo[v][c]);
+        }
+    }
+}
+
+static void c32_output(const c32grid *g, double *prim_out, double *cons_out)
+{
+    int c, v, ke = (g->dim == 1 && g->order == 1) ? 0 : 1;
+    for (c = 0; c < g->n; c++) {
+        double u[4], p[4];
+        for (v = 0; v < g->nv; v++) u[v] = (double)g->hi[v][c] + (double)g->lo[v][c];
+        u2p_f64(g->dim, ke, g->cv64, u, p);
+        for (v = 0; v < g->nv; v++) {
+            if (cons_out) cons_out[(size_t)v * g->n + c] = u[v];
+            if (prim_out) prim_out[(size_t)v * g->n + c] = p[v];
+        }
+    }
+}
+
+static int c32_run(const orc_problem *pb, const double *prim_in, int64_t nsteps, double *prim_out,
+                   double *cons_out, float *hi_out, float *lo_out, double *elapsed)
+{
+    c32grid g;
+    int64_t s;
+    double t0;
+    if (c32_alloc(&g, pb)) return -2;
+    c32_init(&g, prim_in);
+    t0 = orc_now();
+    for (s = 0; s < nsteps; s++) c32_step(&g);
+    if (elapsed) *elapsed = orc_now() - t0;
+    c32_output(&g, prim_out, cons_out);
+    if (hi_out || lo_out) {
+        int v;
+        for (v = 0; v < g.nv; v++) {
+            if (hi_out) memcpy(hi_out + (size_t)v * g.n, g.hi[v], (size_t)g.n * sizeof(float));
+            if (lo_out) memcpy(lo_out + (size_t)v * g.n, g.lo[v], (size_t)g.n * sizeof(float));
+        }
+    }
+    free(g.block);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+int orc_run(const orc_problem *pb, int mode, const double *prim_in, int64_t nsteps,
+            double *prim_out, double *cons_out, int nthreads, double *elapsed_s)
+{
+    if (orc_check(pb) || !prim_in || nsteps < 0) return -1;
+    switch (mode) {
+    case ORC_REF32: return run_r32(pb, prim_in, nsteps, prim_out, cons_out, nthreads, elapsed_s);
+    case ORC_F64: return run_f64(pb, prim_in, nsteps, prim_out, cons_out, nthreads, elapsed_s);
+    case ORC_C32: return c32_run(pb, prim_in, nsteps, prim_out, cons_out, NULL, NULL, elapsed_s);
+    default: return -1;
+    }
+}
+
+int orc_run_c32_raw(const orc_problem *pb, const double *prim_in, int64_t nsteps,
+                    float *hi_out, float *lo_out)
+{
+    if (orc_check(pb) || !prim_in || nsteps < 0) return -1;
+    return c32_run(pb, prim_in, nsteps, NULL, NULL, hi_out, lo_out, NULL);
+}
+
+void orc_constants(const orc_problem *pb, int mode, double out[6])
+{
+    if (mode == ORC_REF32) {
+        cst_r32 k;
+        make_cst_r32(&k, pb->nx, pb->ny, pb->lx, pb->dim == 2 ? pb->ly : 1.0);
+        out[0] = k.gamma; out[1] = k.R; out[2] = k.cv; out[3] = k.dt; out[4] = k.dtdx; out[5] = k.dtdy;
+    } else {
+        cst_f64 k;
+        make_cst_f64(&k, pb->nx, pb->ny, pb->lx, pb->dim == 2 ? pb->ly : 1.0);
+        out[0] = k.gamma; out[1] = k.R; out[2] = k.cv; out[3] = k.dt; out[4] = k.dtdx; out[5] = k.dtdy;
+    }
+}
+
+/* Stage evaluation on whole arrays (Compute_U_from_P, Compute_P_from_U, Compute_F_from_P once). */
+#define STAGE_BODY(SUF)                                                                        \
+    do {                                                                                       \
+        grid##SUF g;                                                                           \
+        int v, c;                                                                              \
+        if (grid_alloc##SUF(&g, pb)) return -2;                                                \
+        for (v = 0; v < g.nv; v++)                                                             \
+            for (c = 0; c < g.n; c++) g.p[v][c] = prim_in[(size_t)v * g.n + c];                \
+        u_from_p##SUF(&g);                                                                     \
+        p_from_u##SUF(&g);                                                                     \
+        flux##SUF(&g);                                                                         \
+        for (v = 0; v < g.nv; v++)                                                             \
+            for (c = 0; c < g.n; c++) {                                                        \
+                size_t o = (size_t)v * g.n + c;                                                \
+                if (cons) cons[o] = g.u[v][c];                                                 \
+                if (prim) prim[o] = g.p[v][c];                                                 \
+                if (fp) fp[o] = g.fp[v][c];                                                    \
+                if (fm) fm[o] = g.fm[v][c];                                                    \
+                if (hp && g.dim == 2) hp[o] = g.hp[v][c];                                      \
+                if (hm && g.dim == 2) hm[o] = g.hm[v][c];                                      \
+            }                                                                                  \
+        if (a)                                                                                 \
+            for (c = 0; c < g.n; c++) a[c] = g.a[c];                                           \
+        free(g.block);                                                                         \
+    } while (0)
+
+int orc_stage(const orc_problem *pb, int mode, const double *prim_in, double *cons, double *prim,
+              double *a, double *fp, double *fm, double *hp, double *hm)
+{
+    if (orc_check(pb) || !prim_in) return -1;
+    if (mode == ORC_REF32) {
+        STAGE_BODY(_r32);
+    } else if (mode == ORC_F64) {
+        STAGE_BODY(_f64);
+    } else if (mode == ORC_C32) {
+        c32grid g;
+        int v, c;
+        if (c32_alloc(&g, pb)) return -2;
+        c32_init(&g, prim_in);
+        for (c = 0; c < g.n; c++) c32_cell_flux(&g, c);
+        for (v = 0; v < g.nv; v++)
+            for (c = 0; c < g.n; c++) {
+                size_t o = (size_t)v * g.n + c;
+                if (cons) cons[o] = (double)g.hi[v][c] + (double)g.lo[v][c];
+                if (fp) fp[o] = g.fp[v][c];
+                if (fm) fm[o] = g.fm[v][c];
+                if (hp && g.dim == 2) hp[o] = g.hp[v][c];
+                if (hm && g.dim == 2) hm[o] = g.hm[v][c];
+            }
+        if (prim) c32_output(&g, prim, NULL);
+        (void)a;
+        free(g.block);
+    } else {
+        return -1;
+    }
+    return 0;
+}
+#undef STAGE_BODY
+
+double orc_minmod(int mode, double l, double r)
+{
+    if (mode == ORC_REF32) return (double)mm_r32((float)l, (float)r);
+    if (mode == ORC_C32) return (double)mm_c32((float)l, (float)r);
+    return mm_f64(l, r);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Initial conditions.  Values pass through float exactly as the reference's  */
+/* float arrays hold them.                                                    */
+/* ------------------------------------------------------------------------- */
+
+/* base_shll.c:54-60:  if (i < 0.5*N) left else right  (comparison in double) */
+void orc_ic_tube(int n, double x0, double rhoL, double uL, double TL,
+                 double rhoR, double uR, double TR, double *prim)
+{
+    int i;
+    for (i = 0; i < n; i++) {
+        int left = (double)i < x0 * (double)n;
+        prim[i] = (double)(float)(left ? rhoL : rhoR);
+        prim[n + i] = (double)(float)(left ? uL : uR);
+        prim[2 * n + i] = (double)(float)(left ? TL : TR);
+    }
+}
+
+/* base_shll_2d.c:93-103: rho = 1 where (i > 0.2*NX) && (i < 0.8*NX) && (j > 0.2*NY) && (j < 0.8*NY),
+ * else 10; u = v = 0; T = 1. */
+void orc_ic_implosion(int nx, int ny, double *prim)
+{
+    int i, j;
+    size_t n = (size_t)nx * ny;
+    for (i = 0; i < nx; i++)
+        for (j = 0; j < ny; j++) {
+            size_t c = (size_t)i * ny + j;
+            int in = (i > 0.2 * nx) && (i < 0.8 * nx) && (j > 0.2 * ny) && (j < 0.8 * ny);
+            prim[c] = in ? 1.0 : 10.0;
+            prim[n + c] = 0.0;
+            prim[2 * n + c] = 0.0;
+            prim[3 * n + c] = 1.0;
+        }
+}
+
+/* 2nd_order_base_shll.c:133-148 (four shocks).  T = p/(rho*R) is evaluated in double
+ * from the float rho (the literals are double), then stored as float.  Cells on the
+ * lines i == 0.75*NX or j == 0.75*NY fall through to the last branch. */
+void orc_ic_four_shocks(int nx, int ny, double *prim)
+{
+    int i, j;
+    size_t n = (size_t)nx * ny;
+    const float R = 1.0f;
+    for (i = 0; i < nx; i++)
+        for (j = 0; j < ny; j++) {
+            size_t c = (size_t)i * ny + j;
+            float r, vx, vy, T;
+            if ((i < 0.75 * nx) && (j < 0.75 * ny)) {
+                r = 0.138f; vx = 1.206f; vy = 1.206f; T = (float)(0.029 / (double)(r * R));
+            } else if ((i > 0.75 * nx) && (j < 0.75 * ny)) {
+                r = 0.5323f; vx = 0.0f; vy = 1.206f; T = (float)(0.3 / (double)(r * R));
+            } else if ((i < 0.75 * nx) && (j > 0.75 * ny)) {
+                r = 0.5323f; vx = 1.206f; vy = 0.0f; T = (float)(0.3 / (double)(r * R));
+            } else {
+                r = 1.5f; vx = 0.0f; vy = 0.0f; T = (float)(1.5 / (double)(r * R));
+            }
+            prim[c] = r; prim[n + c] = vx; prim[2 * n + c] = vy; prim[3 * n + c] = T;
+        }
+}
+
+void orc_ic_circle(int nx, int ny, double lx, double ly, double cx, double cy, double r,
+                   double rho_in, double T_in, double rho_out, double T_out, double *prim)
+{
+    int i, j;
+    size_t n = (size_t)nx * ny;
+    double dx = lx / nx, dy = ly / ny;
+    for (i = 0; i < nx; i++)
+        for (j = 0; j < ny; j++) {
+            size_t c = (size_t)i * ny + j;
+            double x = (i + 0.5) * dx - cx, y = (j + 0.5) * dy - cy;
+            int in = x * x + y * y < r * r;
+            prim[c] = (double)(float)(in ? rho_in : rho_out);
+            prim[n + c] = 0.0;
+            prim[2 * n + c] = 0.0;
+            prim[3 * n + c] = (double)(float)(in ? T_in : T_out);
+        }
+}

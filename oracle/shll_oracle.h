@@ -1,0 +1,114 @@
+/*
+ * shll_oracle.h -- CPU restatement of the SHLL finite-volume time-step loop.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP
+ * product library (libshll_amd.so).  Only tests/, __graft_entry__.smoke()
+ * and bench.py's cpu_baseline leg may load it.  Nothing in the product
+ * links against it or falls back to it.
+ *
+ * It restates, in independent code, the arithmetic of
+ *   base-c/base_shll.c          (1D, first order, reflective walls)
+ *   base-c/base_shll_2d.c       (2D, first order, reflective walls)
+ *   base-c/2nd_order_base_shll.c(2D, second order minmod, outflow)
+ * and a derived 1D second-order scheme (the x sweep of the 2D second-order
+ * file on the 3-variable system, i.e. the 2D file run on y-uniform data with
+ * v = 0).  It keeps the reference's unfused pass structure (one full-array
+ * pass per stage, temporaries for the neighbour gathers) so that it can also
+ * be timed as the CPU baseline, serially or with one OpenMP worksharing loop
+ * per pass (the base-omp pattern).
+ *
+ * Three arithmetic modes:
+ *   ORC_REF32 : float storage, with the double promotions the reference's
+ *               double literals and double sqrt() cause.  Pinned bitwise to
+ *               the compiled reference (tests/golden).
+ *   ORC_F64   : the same association with every value in double.  This is
+ *               the FP64 oracle.
+ *   ORC_C32   : float flux arithmetic with a compensated (hi, lo) float state
+ *               (TwoSum update).  Restates the product's FP32-compensated
+ *               kernels so they can be checked bitwise; its accuracy is
+ *               judged against ORC_F64.
+ *
+ * Parity pinning status (which check fixes which function) is listed per
+ * function in shll_oracle.c.
+ */
+#ifndef SHLL_ORACLE_H
+#define SHLL_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_F64 = 0, ORC_REF32 = 1, ORC_C32 = 2 };
+enum { ORC_BC_REFLECT = 0, ORC_BC_OUTFLOW = 1 };
+
+typedef struct {
+    int32_t dim;    /* 1 or 2 */
+    int32_t order;  /* 1 or 2 */
+    int32_t bc;     /* ORC_BC_REFLECT (first-order files) / ORC_BC_OUTFLOW (second-order file) */
+    int32_t nx;     /* cells in x */
+    int32_t ny;     /* cells in y (1 in 1D) */
+    int32_t pad;
+    double lx;      /* domain length in x (reference: L = 1) */
+    double ly;      /* domain length in y (reference: H = 1) */
+} orc_problem;
+
+/* Number of variables: 3 in 1D (rho, rho*u, E), 4 in 2D (rho, rho*u, rho*v, E). */
+int orc_nvar(const orc_problem *pb);
+
+/* Number of steps the reference's float time loop takes:
+ *   float time = 0; while (time < t_end) { ...; time += DT; }
+ * with DT = (CFL/(R+1))*DX computed in float (base_shll.c:83, :200-218). */
+int64_t orc_steps_to_time(const orc_problem *pb, float t_end);
+
+/* Time step constants of each mode, as doubles (diagnostics / tests). */
+void orc_constants(const orc_problem *pb, int mode, double out[6] /* gamma,R,cv,dt,dt/dx,dt/dy */);
+
+/*
+ * Run nsteps of the scheme from primitive initial data.
+ *   prim_in  : SoA doubles, nvar planes of nx*ny: rho, u, (v), T  (index i*ny+j)
+ *              For ORC_REF32 every value must be exactly representable in float.
+ *   prim_out : SoA doubles, same layout: rho, u, (v), T at the end
+ *              (REF32: the float arrays widened; C32: FP64 primitives of hi+lo)
+ *   cons_out : SoA doubles, nvar planes of conserved variables
+ *              (C32: (double)hi + (double)lo), may be NULL
+ *   nthreads : 1 = serial; >1 = one OpenMP worksharing loop per pass
+ *   elapsed_s: wall time of the step loop only (CLOCK_MONOTONIC), may be NULL
+ * Returns 0 on success, nonzero on invalid arguments / allocation failure.
+ */
+int orc_run(const orc_problem *pb, int mode, const double *prim_in, int64_t nsteps,
+            double *prim_out, double *cons_out, int nthreads, double *elapsed_s);
+
+/* C32 only: also return the raw hi and lo float planes (nvar*nx*ny each). */
+int orc_run_c32_raw(const orc_problem *pb, const double *prim_in, int64_t nsteps,
+                    float *hi_out, float *lo_out);
+
+/* Stage evaluation on whole arrays: the reference's Compute_U_from_P, Compute_P_from_U and
+ * Compute_F_from_P applied once to prim_in (base_shll.c:74-177).  Any output may be NULL.
+ *   cons : nvar planes (C32: hi+lo),  prim : nvar planes after the P<-U pass,
+ *   a    : sound speed (one plane; not produced in C32),
+ *   fp/fm: x split fluxes (nvar planes), hp/hm: y split fluxes (2D only). */
+int orc_stage(const orc_problem *pb, int mode, const double *prim_in, double *cons, double *prim,
+              double *a, double *fp, double *fm, double *hp, double *hm);
+
+/* minmod of the second-order file, in float (REF32) or double (F64). */
+double orc_minmod(int mode, double l, double r);
+
+/* ---- initial conditions (primitive SoA doubles, exact floats) ---- */
+/* Sod tube: cells with i < x0*N get (rhoL,uL,TL) else (rhoR,uR,TR).
+ * The reference's test is  (i < 0.5*N)  in double (base_shll.c:55). */
+void orc_ic_tube(int n, double x0, double rhoL, double uL, double TL,
+                 double rhoR, double uR, double TR, double *prim);
+/* Square implosion of base_shll_2d.c:93-103 (rho=1 inside (0.2,0.8)^2 by index test, 10 outside). */
+void orc_ic_implosion(int nx, int ny, double *prim);
+/* Four shocks of 2nd_order_base_shll.c:133-148. */
+void orc_ic_four_shocks(int nx, int ny, double *prim);
+/* Circle: rho_in for (x-cx)^2+(y-cy)^2 < r^2 at cell centres ((i+.5)/nx*lx,(j+.5)/ny*ly). */
+void orc_ic_circle(int nx, int ny, double lx, double ly, double cx, double cy, double r,
+                   double rho_in, double T_in, double rho_out, double T_out, double *prim);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
