@@ -86,9 +86,12 @@ int64_t orc_steps_to_time(const orc_problem *pb, float t_end)
 
 /* ------------------------------------------------------------------------- */
 /* FP32-compensated mode.                                                    */
-/* Constants are the FP64 oracle's, rounded once to float.  Fluxes are float  */
-/* functions of the hi word; the increment is added to (hi, lo) with TwoSum   */
-/* (Knuth, TAOCP vol. 2, 4.2.2), so hi+lo carries ~48 significant bits.       */
+/* Constants are the FP64 oracle's, rounded once to float.  Split fluxes are  */
+/* float functions of the hi word.  The update is regrouped into face terms   */
+/* t = lam*G (one float per face, shared by both cells) that are added to the */
+/* (hi, lo) pair with TwoSum (Knuth, TAOCP vol. 2, 4.2.2), so the state keeps */
+/* ~48 significant bits and sum(hi+lo) telescopes: mass and energy are        */
+/* conserved to ~1e-14 relative with any wall treatment that conserves them.  */
 /* ------------------------------------------------------------------------- */
 typedef struct {
     int nx, ny, n, nv, dim, order, bc;
@@ -246,31 +249,34 @@ static float c32_slope(const float *q, int c, int s, int stride, int len)
     return mm_c32(q[c] - q[c - stride], q[c + stride] - q[c]);
 }
 
-/* First- and second-order increments along one direction, same gather rules as
- * the reference modes (gather_cell / slope_cell of oracle_scheme.inc). */
-static void c32_dir(const c32grid *g, float **qp, float **qm, const float *sgn, int v, int c, int s,
-                    int stride, int len, float lam, float hlam, float *d1, float *d2)
+/*
+ * Face term t_k = lam * G_k of face k on a line of len cells (face k lies between
+ * cells k-1 and k; c0 is the cell of line position 0).  G is the reference's update
+ * regrouped by faces, which telescopes:
+ *   interior      G_k = (fp_{k-1} + fm_k) [+ 0.5*(dfp_{k-1} - dfm_k)  second order]
+ *   wall, k = 0   G_0 = fm_0 + S*fm_0          (reflective: L = S*fm_0)
+ *                 G_0 = fp_0 + fm_0            (outflow:    L = fp_0)
+ *   wall, k = len G_len = fp_{len-1} + S*fp_{len-1}  /  fp_{len-1} + fm_{len-1}
+ * so that u_s -= lam*(((fp-fm)+R)-L) [- 0.5*lam*(((dfp+dfm)-Rdf)-Ldf)] equals
+ * u_s += t_s - t_{s+1} in exact arithmetic.
+ */
+static float c32_face(const c32grid *g, float **qp, float **qm, const float *sgn, int v, int c0, int k,
+                      int stride, int len, float lam)
 {
-    float Lv, Rv;
-    if (s == 0) {
-        Lv = g->bc == ORC_BC_REFLECT ? sgn[v] * qm[v][c] : qp[v][c];
-        Rv = qm[v][c + stride];
-    } else if (s == len - 1) {
-        Lv = qp[v][c - stride];
-        Rv = g->bc == ORC_BC_REFLECT ? sgn[v] * qp[v][c] : qm[v][c];
+    float G;
+    const float *p = qp[v], *m = qm[v];
+    if (k == 0) {
+        G = g->bc == ORC_BC_REFLECT ? m[c0] + sgn[v] * m[c0] : p[c0] + m[c0];
+    } else if (k == len) {
+        int c = c0 + (len - 1) * stride;
+        G = g->bc == ORC_BC_REFLECT ? p[c] + sgn[v] * p[c] : p[c] + m[c];
     } else {
-        Lv = qp[v][c - stride];
-        Rv = qm[v][c + stride];
+        int cl = c0 + (k - 1) * stride, cr = cl + stride;
+        G = p[cl] + m[cr];
+        if (g->order == 2)
+            G = G + 0.5f * (c32_slope(p, cl, k - 1, stride, len) - c32_slope(m, cr, k, stride, len));
     }
-    *d1 = lam * (((qp[v][c] - qm[v][c]) + Rv) - Lv);
-    *d2 = 0.0f;
-    if (g->order == 2) {
-        float dp = c32_slope(qp[v], c, s, stride, len);
-        float dm = c32_slope(qm[v], c, s, stride, len);
-        float Ldf = s == 0 ? 0.0f : c32_slope(qp[v], c - stride, s - 1, stride, len);
-        float Rdf = s == len - 1 ? 0.0f : c32_slope(qm[v], c + stride, s + 1, stride, len);
-        *d2 = hlam * (((dp + dm) - Rdf) - Ldf);
-    }
+    return lam * G;
 }
 
 static void c32_step(c32grid *g)
@@ -280,21 +286,24 @@ static void c32_step(c32grid *g)
     for (c = 0; c < g->n; c++) {
         int i = g->dim == 1 ? c : c / g->ny;
         int j = g->dim == 1 ? 0 : c % g->ny;
+        int sx = g->dim == 1 ? 1 : g->ny;
+        int cx0 = c - i * sx;      /* line position 0 of this cell's x line */
+        int cy0 = c - j;           /* line position 0 of this cell's y line */
         for (v = 0; v < g->nv; v++) {
-            float dx1, dx2, dy1 = 0.0f, dy2 = 0.0f, y;
-            c32_dir(g, g->fp, g->fm, g->dim == 1 ? sx1_c32 : sx2_c32, v, c, i,
-                    g->dim == 1 ? 1 : g->ny, g->nx, g->lamx, g->hlamx, &dx1, &dx2);
-            if (g->dim == 2)
-                c32_dir(g, g->hp, g->hm, sy2_c32, v, c, j, 1, g->ny, g->lamy, g->hlamy, &dy1, &dy2);
-            y = g->lo[v][c] - dx1;
-            if (g->order == 2) y = y - dx2;
+            /* the fluxes of this step all come from the old hi, so (hi, lo) of this
+             * cell can be overwritten in place */
+            float a = g->hi[v][c], b = g->lo[v][c], e;
+            float tl = c32_face(g, g->fp, g->fm, g->dim == 1 ? sx1_c32 : sx2_c32, v, cx0, i, sx, g->nx, g->lamx);
+            float tr = c32_face(g, g->fp, g->fm, g->dim == 1 ? sx1_c32 : sx2_c32, v, cx0, i + 1, sx, g->nx, g->lamx);
+            two_sum(a, tl, &a, &e); b = b + e;
+            two_sum(a, -tr, &a, &e); b = b + e;
             if (g->dim == 2) {
-                y = y - dy1;
-                if (g->order == 2) y = y - dy2;
+                float tb = c32_face(g, g->hp, g->hm, sy2_c32, v, cy0, j, 1, g->ny, g->lamy);
+                float tt = c32_face(g, g->hp, g->hm, sy2_c32, v, cy0, j + 1, 1, g->ny, g->lamy);
+                two_sum(a, tb, &a, &e); b = b + e;
+                two_sum(a, -tt, &a, &e); b = b + e;
             }
-            /* the new fluxes of this step were all computed from the old hi above,
-             * so the state can be overwritten in place */
-            two_sum(g->hi[v][c], y, &g->hi[v][c], &g->lo[v][c]);
+            two_sum(a, b, &g->hi[v][c], &g->lo[v][c]);
         }
     }
 }

@@ -1,0 +1,135 @@
+// shll_device.h -- device arithmetic of the SHLL step for gfx950.
+//
+// Three precisions share one set of templates:
+//   PREC_F64   : every value double, the reference association (FP64 oracle form)
+//   PREC_REF32 : float values with the reference's double promotions, so results are
+//                bitwise those of the reference C on x86-64
+//   PREC_C32   : float arithmetic, (hi, lo) float state, conservative face update
+//                with TwoSum (the FP32-compensated form)
+//
+// The library is compiled with -ffp-contract=off: every + - * / below is one
+// IEEE-754 operation in source order, '/' and sqrt are correctly rounded
+// (-fhip-fp32-correctly-rounded-divide-sqrt, IEEE f64 div/sqrt), so the F64 and
+// REF32 kernels reproduce the CPU results bit for bit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "shll_types.h"
+
+namespace shll {
+
+// ---------------------------------------------------------------------------
+// cross-lane moves inside one wavefront (DPP wave_shr:1 / wave_shl:1)
+// from_left(x)  : the value x of lane-1 (lane 0 keeps its own)
+// from_right(x) : the value x of lane+1 (lane 63 keeps its own)
+// Every lane of the wave executes these with the same control flow.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int dpp_from_left(int x) {
+    return __builtin_amdgcn_update_dpp(x, x, 0x138, 0xf, 0xf, false);
+}
+__device__ __forceinline__ int dpp_from_right(int x) {
+    return __builtin_amdgcn_update_dpp(x, x, 0x130, 0xf, 0xf, false);
+}
+__device__ __forceinline__ float from_left(float x) {
+    return __int_as_float(dpp_from_left(__float_as_int(x)));
+}
+__device__ __forceinline__ float from_right(float x) {
+    return __int_as_float(dpp_from_right(__float_as_int(x)));
+}
+__device__ __forceinline__ double from_left(double x) {
+    int lo = __double2loint(x), hi = __double2hiint(x);
+    return __hiloint2double(dpp_from_left(hi), dpp_from_left(lo));
+}
+__device__ __forceinline__ double from_right(double x) {
+    int lo = __double2loint(x), hi = __double2hiint(x);
+    return __hiloint2double(dpp_from_right(hi), dpp_from_right(lo));
+}
+
+// wave-uniform value (moves it to a scalar register)
+__device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
+// ---------------------------------------------------------------------------
+// Reference association (F64 and REF32).  RT is the storage/arithmetic type;
+// (double) conversions sit where the reference's double literals and its double
+// sqrt() put them (base_shll.c:135-139, :172-175; base_shll_2d.c:313-317).
+// For RT = double every conversion is the identity.
+// ---------------------------------------------------------------------------
+template <class RT>
+struct RefArith {
+    // Compute_P_from_U, 1D.  ke = 0: (0.5*u)*u (base_shll.c:174); ke = 1: 0.5*(u*u)
+    // (the 2D files' association, used by the derived 1D second-order scheme).
+    __device__ static __forceinline__ void prim1(RT u0, RT u1, RT u2, int ke, RT cv, RT gR, RT& vx, RT& T,
+                                                 RT& a) {
+        vx = u1 / u0;
+        double kin = ke == 0 ? (0.5 * (double)vx) * (double)vx : 0.5 * (double)(vx * vx);
+        T = (RT)(((double)(u2 / u0) - kin) / (double)cv);
+        a = (RT)sqrt((double)(gR * T));
+    }
+    // Compute_P_from_U, 2D (base_shll_2d.c:313-317)
+    __device__ static __forceinline__ void prim2(RT u0, RT u1, RT u2, RT u3, RT cv, RT gR, RT& vx, RT& vy,
+                                                 RT& T, RT& a) {
+        vx = u1 / u0;
+        vy = u2 / u0;
+        T = (RT)(((double)(u3 / u0) - 0.5 * (double)(vx * vx + vy * vy)) / (double)cv);
+        a = (RT)sqrt((double)(gR * T));
+    }
+    // Z invariants (base_shll.c:137-139)
+    __device__ static __forceinline__ void zinv(RT M, RT a, RT& z1, RT& z2, RT& z3) {
+        z1 = (RT)(0.5 * ((double)M + 1.0));
+        z2 = (RT)((0.5 * (double)a) * (1.0 - (double)(M * M)));
+        z3 = (RT)(0.5 * ((double)M - 1.0));
+    }
+    // minmod (2nd_order_base_shll.c:191-201): product sign test, ties -> right
+    __device__ static __forceinline__ RT mm(RT l, RT r) {
+        RT pick = fabs((double)l) < fabs((double)r) ? l : r;
+        return (double)(l * r) < 0.0 ? (RT)0 : pick;
+    }
+};
+
+// FP32-compensated arithmetic: plain float, same formula shapes.
+struct C32Arith {
+    __device__ static __forceinline__ void prim1(float u0, float u1, float u2, float rcv, float gR, float& vx,
+                                                 float& T, float& a) {
+        float inv = 1.0f / u0;
+        vx = u1 * inv;
+        float e = u2 * inv;
+        T = (e - 0.5f * (vx * vx)) * rcv;
+        a = sqrtf(gR * T);
+    }
+    __device__ static __forceinline__ void prim2(float u0, float u1, float u2, float u3, float rcv, float gR,
+                                                 float& vx, float& vy, float& T, float& a) {
+        float inv = 1.0f / u0;
+        vx = u1 * inv;
+        vy = u2 * inv;
+        float e = u3 * inv;
+        T = (e - 0.5f * (vx * vx + vy * vy)) * rcv;
+        a = sqrtf(gR * T);
+    }
+    __device__ static __forceinline__ void zinv(float M, float a, float& z1, float& z2, float& z3) {
+        z1 = 0.5f * (M + 1.0f);
+        z2 = (0.5f * a) * (1.0f - M * M);
+        z3 = 0.5f * (M - 1.0f);
+    }
+    __device__ static __forceinline__ float mm(float l, float r) {
+        float pick = fabsf(l) < fabsf(r) ? l : r;
+        return l * r < 0.0f ? 0.0f : pick;
+    }
+};
+
+// TwoSum (Knuth, TAOCP vol. 2, 4.2.2): s + e == a + b exactly
+__device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
+    s = a + b;
+    float bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+
+// SHLL split of one flux component: F+ = F*Z1 + U*Z2, F- = -F*Z3 - U*Z2 (base_shll.c:149-157)
+template <class RT>
+__device__ __forceinline__ void split(RT f, RT u, RT z1, RT z2, RT z3, RT& fp, RT& fm) {
+    fp = f * z1 + u * z2;
+    fm = -f * z3 - u * z2;
+}
+
+}  // namespace shll

@@ -1,0 +1,271 @@
+// shll_k1d.hip -- fused 1D SHLL step kernels (first and second order) for gfx950.
+//
+// One wavefront owns a contiguous segment of 64*C cells, C consecutive cells per
+// lane, held in registers for the whole launch.  A step is
+//   primitive recovery + SHLL split fluxes of the lane's C cells   (Compute_P_from_U,
+//                                                                   Compute_F_from_P)
+//   neighbour fluxes of the two adjacent lanes by DPP wave shifts   (the gather of
+//                                                                   Update_U_from_F)
+//   boundary cells and the conservative update in registers         (Update_U_from_F)
+// so a launch of k steps reads the state from HBM once and writes it once; the
+// segment's outer order*k cells on each side are recomputed halo (temporal
+// blocking without LDS and without barriers: wavefronts never wait on each other).
+//
+// References: base_shll.c:87-177 (first order, reflective), 2nd_order_base_shll.c
+// :203-459 (second order, outflow; its x sweep on the 3-variable system is the 1D
+// second-order scheme).
+#include "shll_device.h"
+
+namespace shll {
+
+template <int PREC> struct Real { using T = double; };
+template <> struct Real<PREC_REF32> { using T = float; };
+template <> struct Real<PREC_C32> { using T = float; };
+
+// slope of a split-flux window at position p (window index p+2) of global cell gg;
+// zero on the two boundary cells (2nd_order_base_shll.c:227-256)
+template <class A, class RT, int NW>
+__device__ __forceinline__ RT slope_at(const RT (&W)[NW], int p, int gg, int n) {
+    RT s = A::mm(W[p + 2] - W[p + 1], W[p + 3] - W[p + 2]);
+    return (gg <= 0 || gg >= n - 1) ? (RT)0 : s;
+}
+
+// window of a per-lane array q[C]: W[p+2] = value at lane position p, p in [-2, C+1]
+template <class RT, int C>
+__device__ __forceinline__ void make_window(const RT (&q)[C], RT (&W)[C + 4]) {
+#pragma unroll
+    for (int m = 0; m < C; ++m) W[m + 2] = q[m];
+    if constexpr (C >= 2) {
+        W[0] = from_left(q[C - 2]);
+        W[1] = from_left(q[C - 1]);
+        W[C + 2] = from_right(q[0]);
+        W[C + 3] = from_right(q[1]);
+    } else {
+        W[1] = from_left(q[0]);
+        W[0] = from_left(W[1]);
+        W[3] = from_right(q[0]);
+        W[4] = from_right(W[3]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// F64 / REF32: the reference association, cell by cell
+// ---------------------------------------------------------------------------
+template <int PREC, int ORDER, int C>
+__device__ __forceinline__ void step_ref(typename Real<PREC>::T (&u)[3][C], const Inst& in, int gl) {
+    using RT = typename Real<PREC>::T;
+    using A = RefArith<RT>;
+    const RT cv = PREC == PREC_F64 ? (RT)in.k.d_cv : (RT)in.k.f_cv;
+    const RT gR = PREC == PREC_F64 ? (RT)in.k.d_gR : (RT)in.k.f_gR;
+    const RT R = PREC == PREC_F64 ? (RT)in.k.d_R : (RT)in.k.f_R;
+    const RT lam = PREC == PREC_F64 ? (RT)in.k.d_lamx : (RT)in.k.f_lamx;
+    const int n = in.nx;
+    RT fp[3][C], fm[3][C];
+#pragma unroll
+    for (int m = 0; m < C; ++m) {
+        RT vx, T, a, z1, z2, z3;
+        A::prim1(u[0][m], u[1][m], u[2][m], in.ke, cv, gR, vx, T, a);
+        A::zinv(vx / a, a, z1, z2, z3);
+        RT P = u[0][m] * R * T;
+        RT f0 = u[1][m], f1 = u[1][m] * vx + P, f2 = vx * (u[2][m] + P);
+        split<RT>(f0, u[0][m], z1, z2, z3, fp[0][m], fm[0][m]);
+        split<RT>(f1, u[1][m], z1, z2, z3, fp[1][m], fm[1][m]);
+        split<RT>(f2, u[2][m], z1, z2, z3, fp[2][m], fm[2][m]);
+    }
+    const bool refl = in.bc == BC_REFLECT;
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+        RT FP[C + 4], FM[C + 4];
+        make_window<RT, C>(fp[v], FP);
+        make_window<RT, C>(fm[v], FM);
+        const RT sgn = v == 1 ? (RT)1 : (RT)-1;  // reflective wall signs (base_shll.c:95-110)
+#pragma unroll
+        for (int m = 0; m < C; ++m) {
+            const int g = gl + m;
+            RT Lv = g == 0 ? (refl ? sgn * fm[v][m] : fp[v][m]) : FP[m + 1];
+            RT Rv = g == n - 1 ? (refl ? sgn * fp[v][m] : fm[v][m]) : FM[m + 3];
+            RT uu = u[v][m] - lam * (fp[v][m] - fm[v][m] + Rv - Lv);
+            if constexpr (ORDER == 2) {
+                RT dp = slope_at<A>(FP, m, g, n);
+                RT dm = slope_at<A>(FM, m, g, n);
+                RT Ldf = g == 0 ? (RT)0 : slope_at<A>(FP, m - 1, g - 1, n);
+                RT Rdf = g == n - 1 ? (RT)0 : slope_at<A>(FM, m + 1, g + 1, n);
+                uu = (RT)((double)uu - 0.5 * (double)lam * (double)(dp + dm - Rdf - Ldf));
+            }
+            u[v][m] = uu;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// C32: float fluxes of the hi word, face terms t = lam*G added with TwoSum
+// ---------------------------------------------------------------------------
+template <int ORDER, int C>
+__device__ __forceinline__ void step_c32(float (&hi)[3][C], float (&lo)[3][C], const Inst& in, int gl) {
+    using A = C32Arith;
+    const float rcv = in.k.c_rcv, gR = in.k.c_gR, R = in.k.c_R, lam = in.k.c_lamx;
+    const int n = in.nx;
+    float fp[3][C], fm[3][C];
+#pragma unroll
+    for (int m = 0; m < C; ++m) {
+        float vx, T, a, z1, z2, z3;
+        A::prim1(hi[0][m], hi[1][m], hi[2][m], rcv, gR, vx, T, a);
+        A::zinv(vx / a, a, z1, z2, z3);
+        float P = hi[0][m] * R * T;
+        float f0 = hi[1][m], f1 = hi[1][m] * vx + P, f2 = vx * (hi[2][m] + P);
+        split<float>(f0, hi[0][m], z1, z2, z3, fp[0][m], fm[0][m]);
+        split<float>(f1, hi[1][m], z1, z2, z3, fp[1][m], fm[1][m]);
+        split<float>(f2, hi[2][m], z1, z2, z3, fp[2][m], fm[2][m]);
+    }
+    const bool refl = in.bc == BC_REFLECT;
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+        float FP[C + 4], FM[C + 4], t[C + 1];
+        make_window<float, C>(fp[v], FP);
+        make_window<float, C>(fm[v], FM);
+        const float sgn = v == 1 ? 1.0f : -1.0f;
+#pragma unroll
+        for (int f = 0; f <= C; ++f) {
+            const int gf = gl + f;  // face gf lies between cells gf-1 and gf
+            float G = FP[f + 1] + FM[f + 2];
+            if constexpr (ORDER == 2)
+                G = G + 0.5f * (slope_at<A>(FP, f - 1, gf - 1, n) - slope_at<A>(FM, f, gf, n));
+            float Gl = refl ? FM[f + 2] + sgn * FM[f + 2] : FP[f + 2] + FM[f + 2];
+            float Gr = refl ? FP[f + 1] + sgn * FP[f + 1] : FP[f + 1] + FM[f + 1];
+            G = gf == 0 ? Gl : (gf == n ? Gr : G);
+            t[f] = lam * G;
+        }
+#pragma unroll
+        for (int m = 0; m < C; ++m) {
+            float a = hi[v][m], b = lo[v][m], e;
+            two_sum(a, t[m], a, e);
+            b = b + e;
+            two_sum(a, -t[m + 1], a, e);
+            b = b + e;
+            two_sum(a, b, hi[v][m], lo[v][m]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// the launch: one wavefront per Item1
+// ---------------------------------------------------------------------------
+template <int PREC, int ORDER, int C, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_step1d(StepArgs a) {
+    using RT = typename Real<PREC>::T;
+    const int wave = (int)blockIdx.x * WPB + uniform((int)threadIdx.x >> 6);
+    if (wave >= a.n_items) return;
+    const int lane = (int)threadIdx.x & 63;
+    const Item1 it = ((const Item1*)a.items)[wave];
+    const Inst in = a.inst[it.inst];
+    const int64_t left = in.nsteps - a.step_base;
+    const int ns = left < (int64_t)a.ksteps ? (int)left : a.ksteps;
+    if (ns <= 0) return;
+    const int n = in.nx;
+    const int gl = it.g0 + lane * C;
+    const int64_t P = a.plane;
+
+    if constexpr (PREC == PREC_C32) {
+        const float* __restrict__ ih = (const float*)a.in0 + in.off;
+        const float* __restrict__ il = (const float*)a.in1 + in.off;
+        float hi[3][C], lo[3][C];
+#pragma unroll
+        for (int m = 0; m < C; ++m) {
+            const int g = gl + m;
+            const bool ok = g >= 0 && g < n;
+#pragma unroll
+            for (int v = 0; v < 3; ++v) {
+                hi[v][m] = ok ? ih[v * P + g] : (v == 0 ? 1.0f : (v == 1 ? 0.0f : 2.5f));
+                lo[v][m] = ok ? il[v * P + g] : 0.0f;
+            }
+        }
+        for (int s = 0; s < ns; ++s) step_c32<ORDER, C>(hi, lo, in, gl);
+        float* __restrict__ oh = (float*)a.out0 + in.off;
+        float* __restrict__ ol = (float*)a.out1 + in.off;
+#pragma unroll
+        for (int m = 0; m < C; ++m) {
+            const int g = gl + m;
+            if (g >= it.x0 && g < it.x0 + it.w) {
+#pragma unroll
+                for (int v = 0; v < 3; ++v) {
+                    oh[v * P + g] = hi[v][m];
+                    ol[v * P + g] = lo[v][m];
+                }
+            }
+        }
+    } else {
+        const RT* __restrict__ iu = (const RT*)a.in0 + in.off;
+        RT u[3][C];
+#pragma unroll
+        for (int m = 0; m < C; ++m) {
+            const int g = gl + m;
+            const bool ok = g >= 0 && g < n;
+#pragma unroll
+            for (int v = 0; v < 3; ++v) u[v][m] = ok ? iu[v * P + g] : (RT)(v == 0 ? 1.0 : (v == 1 ? 0.0 : 2.5));
+        }
+        for (int s = 0; s < ns; ++s) step_ref<PREC, ORDER, C>(u, in, gl);
+        RT* __restrict__ ou = (RT*)a.out0 + in.off;
+#pragma unroll
+        for (int m = 0; m < C; ++m) {
+            const int g = gl + m;
+            if (g >= it.x0 && g < it.x0 + it.w) {
+#pragma unroll
+                for (int v = 0; v < 3; ++v) ou[v * P + g] = u[v][m];
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// dispatch
+// ---------------------------------------------------------------------------
+template <int PREC, int ORDER, int C>
+static hipError_t launch_c(int wpb, int nblocks, hipStream_t s, const StepArgs& a) {
+    if (wpb == 1)
+        hipLaunchKernelGGL((k_step1d<PREC, ORDER, C, 1>), dim3(nblocks), dim3(64), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_step1d<PREC, ORDER, C, 4>), dim3(nblocks), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+template <int PREC, int ORDER>
+static hipError_t launch_o(int C, int wpb, int nblocks, hipStream_t s, const StepArgs& a) {
+    switch (C) {
+    case 1: return launch_c<PREC, ORDER, 1>(wpb, nblocks, s, a);
+    case 2: return launch_c<PREC, ORDER, 2>(wpb, nblocks, s, a);
+    case 4: return launch_c<PREC, ORDER, 4>(wpb, nblocks, s, a);
+    case 8: return launch_c<PREC, ORDER, 8>(wpb, nblocks, s, a);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_step1d(int prec, int order, int C, int wpb, int n_items, hipStream_t s, const StepArgs& a) {
+    const int nblocks = (n_items + wpb - 1) / wpb;
+    if (nblocks <= 0) return hipSuccess;
+    if (prec == PREC_F64) return order == 1 ? launch_o<PREC_F64, 1>(C, wpb, nblocks, s, a)
+                                            : launch_o<PREC_F64, 2>(C, wpb, nblocks, s, a);
+    if (prec == PREC_REF32) return order == 1 ? launch_o<PREC_REF32, 1>(C, wpb, nblocks, s, a)
+                                              : launch_o<PREC_REF32, 2>(C, wpb, nblocks, s, a);
+    if (prec == PREC_C32) return order == 1 ? launch_o<PREC_C32, 1>(C, wpb, nblocks, s, a)
+                                            : launch_o<PREC_C32, 2>(C, wpb, nblocks, s, a);
+    return hipErrorInvalidValue;
+}
+
+template <int PREC, int ORDER>
+static const void* fn_o(int C, int wpb) {
+#define SHLL_FN(CC)                                                                              \
+    if (C == CC) return wpb == 1 ? (const void*)k_step1d<PREC, ORDER, CC, 1> : (const void*)k_step1d<PREC, ORDER, CC, 4>;
+    SHLL_FN(1) SHLL_FN(2) SHLL_FN(4) SHLL_FN(8)
+#undef SHLL_FN
+    return nullptr;
+}
+
+const void* step1d_fn(int prec, int order, int C, int wpb) {
+    if (prec == PREC_F64) return order == 1 ? fn_o<PREC_F64, 1>(C, wpb) : fn_o<PREC_F64, 2>(C, wpb);
+    if (prec == PREC_REF32) return order == 1 ? fn_o<PREC_REF32, 1>(C, wpb) : fn_o<PREC_REF32, 2>(C, wpb);
+    return order == 1 ? fn_o<PREC_C32, 1>(C, wpb) : fn_o<PREC_C32, 2>(C, wpb);
+}
+
+bool step1d_supported(int C, int wpb) { return (C == 1 || C == 2 || C == 4 || C == 8) && (wpb == 1 || wpb == 4); }
+
+}  // namespace shll

@@ -1,0 +1,235 @@
+// shll_k2d.hip -- fused 2D SHLL step kernels (first and second order) for gfx950.
+//
+// Layout: SoA planes, cell (i, j) at i*ny + j, y contiguous (base_shll_2d.c:146-221).
+// One wavefront takes a strip of 64 consecutive columns j (lane = j - j0) and streams
+// down the rows i.  For each row it loads the state once (64 consecutive cells per
+// variable: one coalesced access), recovers the primitives, forms the x and y split
+// fluxes, and keeps a rolling window of 2*ORDER+1 rows of x fluxes in registers.
+// Row c = t - ORDER is updated once row t is in: its x neighbours come from the
+// window, its y neighbours from the adjacent lanes (DPP wave shifts).  The outer
+// ORDER lanes of a strip are recomputed halo.  No LDS, no barriers.
+//
+// The update keeps the reference's unsplit order: x term, (second-order x term),
+// y term, (second-order y term), all from the same state (base_shll_2d.c:225-236,
+// 2nd_order_base_shll.c:436-458).  Walls: reflective with per-component signs
+// (base_shll_2d.c:150-216) or outflow (2nd_order_base_shll.c:214-323).
+#include "shll_device.h"
+
+namespace shll {
+
+template <int PREC> struct Real2 { using T = double; };
+template <> struct Real2<PREC_REF32> { using T = float; };
+template <> struct Real2<PREC_C32> { using T = float; };
+
+// x and y split fluxes of one cell.  U: conserved (F64/REF32: the state; C32: hi).
+template <int PREC, class RT>
+__device__ __forceinline__ void fluxes2(const RT (&U)[4], const Inst& in, RT (&xp)[4], RT (&xm)[4], RT (&yp)[4],
+                                        RT (&ym)[4]) {
+    RT vx, vy, T, a, z1, z2, z3;
+    RT R;
+    if constexpr (PREC == PREC_C32) {
+        C32Arith::prim2(U[0], U[1], U[2], U[3], in.k.c_rcv, in.k.c_gR, vx, vy, T, a);
+        R = in.k.c_R;
+    } else {
+        const RT cv = PREC == PREC_F64 ? (RT)in.k.d_cv : (RT)in.k.f_cv;
+        const RT gR = PREC == PREC_F64 ? (RT)in.k.d_gR : (RT)in.k.f_gR;
+        RefArith<RT>::prim2(U[0], U[1], U[2], U[3], cv, gR, vx, vy, T, a);
+        R = PREC == PREC_F64 ? (RT)in.k.d_R : (RT)in.k.f_R;
+    }
+    const RT P = U[0] * R * T;
+    // x direction (base_shll_2d.c:249-272)
+    if constexpr (PREC == PREC_C32) C32Arith::zinv(vx / a, a, z1, z2, z3);
+    else RefArith<RT>::zinv(vx / a, a, z1, z2, z3);
+    split<RT>(U[1], U[0], z1, z2, z3, xp[0], xm[0]);
+    split<RT>(U[1] * vx + P, U[1], z1, z2, z3, xp[1], xm[1]);
+    split<RT>(U[1] * vy, U[2], z1, z2, z3, xp[2], xm[2]);
+    split<RT>(vx * (U[3] + P), U[3], z1, z2, z3, xp[3], xm[3]);
+    // y direction (base_shll_2d.c:275-298)
+    if constexpr (PREC == PREC_C32) C32Arith::zinv(vy / a, a, z1, z2, z3);
+    else RefArith<RT>::zinv(vy / a, a, z1, z2, z3);
+    split<RT>(U[2], U[0], z1, z2, z3, yp[0], ym[0]);
+    split<RT>(U[2] * vx, U[1], z1, z2, z3, yp[1], ym[1]);
+    split<RT>(U[2] * vy + P, U[2], z1, z2, z3, yp[2], ym[2]);
+    split<RT>(vy * (U[3] + P), U[3], z1, z2, z3, yp[3], ym[3]);
+}
+
+// reflective wall signs: x walls (-,+,-,-), y walls (-,-,+,-)
+__device__ __forceinline__ int sgn_x(int v) { return v == 1 ? 1 : -1; }
+__device__ __forceinline__ int sgn_y(int v) { return v == 2 ? 1 : -1; }
+
+template <int PREC, int ORDER, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_step2d(StepArgs a) {
+    using RT = typename Real2<PREC>::T;
+    using A = typename std::conditional<PREC == PREC_C32, C32Arith, RefArith<RT>>::type;
+    constexpr int NW = 2 * ORDER + 1;  // rows of x fluxes kept: c-ORDER .. c+ORDER
+    constexpr int NS = ORDER + 1;      // rows of state / y fluxes kept: c .. c+ORDER
+    constexpr bool CMP = PREC == PREC_C32;
+
+    const int wave = (int)blockIdx.x * WPB + uniform((int)threadIdx.x >> 6);
+    if (wave >= a.n_items) return;
+    const int lane = (int)threadIdx.x & 63;
+    const Item2 it = ((const Item2*)a.items)[wave];
+    const Inst in = a.inst[it.inst];
+    if (in.nsteps - a.step_base <= 0) return;
+    const int nx = in.nx, ny = in.ny;
+    const int64_t P = a.plane;
+    const int j = it.j0 + lane;
+    const bool jok = j >= 0 && j < ny;
+    const bool jown = j >= it.jx0 && j < it.jx0 + it.jw;
+    const bool refl = in.bc == BC_REFLECT;
+    const int jc = jok ? j : (j < 0 ? 0 : ny - 1);  // clamped column for addressing
+
+    const RT* __restrict__ i0p = (const RT*)a.in0 + in.off;
+    const RT* __restrict__ i1p = CMP ? (const RT*)a.in1 + in.off : nullptr;
+    RT* __restrict__ o0p = (RT*)a.out0 + in.off;
+    RT* __restrict__ o1p = CMP ? (RT*)a.out1 + in.off : nullptr;
+
+    RT xp[NW][4], xm[NW][4];   // x split fluxes, window rows
+    RT yp[NS][4], ym[NS][4];   // y split fluxes
+    RT su[NS][4], sl[NS][4];   // state (C32: hi, lo)
+
+#pragma unroll
+    for (int w = 0; w < NW; ++w)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) xp[w][v] = xm[w][v] = (RT)0;
+#pragma unroll
+    for (int w = 0; w < NS; ++w)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) yp[w][v] = ym[w][v] = su[w][v] = sl[w][v] = (RT)0;
+
+    const RT lamx = PREC == PREC_F64 ? (RT)in.k.d_lamx : (PREC == PREC_REF32 ? (RT)in.k.f_lamx : (RT)in.k.c_lamx);
+    const RT lamy = PREC == PREC_F64 ? (RT)in.k.d_lamy : (PREC == PREC_REF32 ? (RT)in.k.f_lamy : (RT)in.k.c_lamy);
+
+    const int tend = it.ix0 + it.iw + ORDER;
+    for (int t = it.ix0 - ORDER; t < tend; ++t) {
+        // rotate the windows by one row
+#pragma unroll
+        for (int w = 0; w + 1 < NW; ++w)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) { xp[w][v] = xp[w + 1][v]; xm[w][v] = xm[w + 1][v]; }
+#pragma unroll
+        for (int w = 0; w + 1 < NS; ++w)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                yp[w][v] = yp[w + 1][v]; ym[w][v] = ym[w + 1][v];
+                su[w][v] = su[w + 1][v]; sl[w][v] = sl[w + 1][v];
+            }
+        if (t >= 0 && t < nx) {
+            const int64_t o = (int64_t)t * ny + jc;
+            RT U[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                U[v] = jok ? i0p[v * P + o] : (RT)(v == 0 ? 1.0 : (v == 3 ? 2.5 : 0.0));
+                su[NS - 1][v] = U[v];
+                if constexpr (CMP) sl[NS - 1][v] = jok ? i1p[v * P + o] : (RT)0;
+            }
+            fluxes2<PREC, RT>(U, in, xp[NW - 1], xm[NW - 1], yp[NS - 1], ym[NS - 1]);
+        }
+        const int c = t - ORDER;  // row to update
+        if (c < it.ix0 || c >= nx) continue;
+        const int64_t o = (int64_t)c * ny + jc;
+
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const RT sx = (RT)sgn_x(v), sy = (RT)sgn_y(v);
+            const RT fpc = xp[ORDER][v], fmc = xm[ORDER][v];
+            const RT hpc = yp[0][v], hmc = ym[0][v];
+            const RT hpl = from_left(hpc), hml = from_left(hmc);
+            const RT hpr = from_right(hpc), hmr = from_right(hmc);
+            // slopes (second order); zero on boundary rows / columns
+            RT dfp = 0, dfm = 0, dfpl = 0, dfmr = 0, dhp = 0, dhm = 0, dhpl = 0, dhmr = 0;
+            if constexpr (ORDER == 2) {
+                dfp = (c <= 0 || c >= nx - 1) ? (RT)0 : A::mm(xp[2][v] - xp[1][v], xp[3][v] - xp[2][v]);
+                dfm = (c <= 0 || c >= nx - 1) ? (RT)0 : A::mm(xm[2][v] - xm[1][v], xm[3][v] - xm[2][v]);
+                dfpl = (c - 1 <= 0 || c - 1 >= nx - 1) ? (RT)0 : A::mm(xp[1][v] - xp[0][v], xp[2][v] - xp[1][v]);
+                dfmr = (c + 1 <= 0 || c + 1 >= nx - 1) ? (RT)0 : A::mm(xm[3][v] - xm[2][v], xm[4][v] - xm[3][v]);
+                dhp = (j <= 0 || j >= ny - 1) ? (RT)0 : A::mm(hpc - hpl, hpr - hpc);
+                dhm = (j <= 0 || j >= ny - 1) ? (RT)0 : A::mm(hmc - hml, hmr - hmc);
+                dhpl = from_left(dhp);
+                dhmr = from_right(dhm);
+            }
+            if constexpr (!CMP) {
+                // reference association
+                RT Lv = c == 0 ? (refl ? sx * fmc : fpc) : xp[ORDER - 1][v];
+                RT Rv = c == nx - 1 ? (refl ? sx * fpc : fmc) : xm[ORDER + 1][v];
+                RT Bv = j == 0 ? (refl ? sy * hmc : hpc) : hpl;
+                RT Tv = j == ny - 1 ? (refl ? sy * hpc : hmc) : hmr;
+                RT uu = su[0][v] - lamx * (fpc - fmc + Rv - Lv);
+                if constexpr (ORDER == 2) {
+                    RT Ldf = c == 0 ? (RT)0 : dfpl;
+                    RT Rdf = c == nx - 1 ? (RT)0 : dfmr;
+                    uu = (RT)((double)uu - 0.5 * (double)lamx * (double)(dfp + dfm - Rdf - Ldf));
+                }
+                uu = uu - lamy * (hpc - hmc + Tv - Bv);
+                if constexpr (ORDER == 2) {
+                    RT Bdf = j == 0 ? (RT)0 : dhpl;
+                    RT Tdf = j == ny - 1 ? (RT)0 : dhmr;
+                    uu = (RT)((double)uu - 0.5 * (double)lamy * (double)(dhp + dhm - Tdf - Bdf));
+                }
+                if (jown && c < it.ix0 + it.iw) o0p[v * P + o] = uu;
+            } else {
+                // face terms (regrouped reference update), TwoSum into (hi, lo)
+                const RT fpl = xp[ORDER - 1][v], fmr = xm[ORDER + 1][v];
+                RT Gl = fpl + fmc, Gr = fpc + fmr;
+                RT Gb = hpl + hmc;
+                if constexpr (ORDER == 2) {
+                    Gl = Gl + 0.5f * (dfpl - dfm);
+                    Gr = Gr + 0.5f * (dfp - dfmr);
+                    Gb = Gb + 0.5f * (dhpl - dhm);
+                }
+                Gl = c == 0 ? (refl ? fmc + sx * fmc : fpc + fmc) : Gl;
+                Gr = c == nx - 1 ? (refl ? fpc + sx * fpc : fpc + fmc) : Gr;
+                Gb = j == 0 ? (refl ? hmc + sy * hmc : hpc + hmc) : Gb;
+                const RT tl = lamx * Gl, tr = lamx * Gr, tb = lamy * Gb;
+                RT tt = from_right(tb);
+                tt = j == ny - 1 ? lamy * (refl ? hpc + sy * hpc : hpc + hmc) : tt;
+                RT aa = su[0][v], bb = sl[0][v], e;
+                two_sum(aa, tl, aa, e);
+                bb = bb + e;
+                two_sum(aa, -tr, aa, e);
+                bb = bb + e;
+                two_sum(aa, tb, aa, e);
+                bb = bb + e;
+                two_sum(aa, -tt, aa, e);
+                bb = bb + e;
+                two_sum(aa, bb, aa, e);
+                if (jown && c < it.ix0 + it.iw) {
+                    o0p[v * P + o] = aa;
+                    o1p[v * P + o] = e;
+                }
+            }
+        }
+    }
+}
+
+template <int PREC, int ORDER>
+static hipError_t launch2_o(int wpb, int nblocks, hipStream_t s, const StepArgs& a) {
+    if (wpb == 1) hipLaunchKernelGGL((k_step2d<PREC, ORDER, 1>), dim3(nblocks), dim3(64), 0, s, a);
+    else hipLaunchKernelGGL((k_step2d<PREC, ORDER, 4>), dim3(nblocks), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+template <int PREC, int ORDER>
+static const void* fn2_o(int wpb) {
+    return wpb == 1 ? (const void*)k_step2d<PREC, ORDER, 1> : (const void*)k_step2d<PREC, ORDER, 4>;
+}
+
+const void* step2d_fn(int prec, int order, int wpb) {
+    if (prec == PREC_F64) return order == 1 ? fn2_o<PREC_F64, 1>(wpb) : fn2_o<PREC_F64, 2>(wpb);
+    if (prec == PREC_REF32) return order == 1 ? fn2_o<PREC_REF32, 1>(wpb) : fn2_o<PREC_REF32, 2>(wpb);
+    return order == 1 ? fn2_o<PREC_C32, 1>(wpb) : fn2_o<PREC_C32, 2>(wpb);
+}
+
+hipError_t launch_step2d(int prec, int order, int wpb, int n_items, hipStream_t s, const StepArgs& a) {
+    const int nblocks = (n_items + wpb - 1) / wpb;
+    if (nblocks <= 0) return hipSuccess;
+    if (prec == PREC_F64) return order == 1 ? launch2_o<PREC_F64, 1>(wpb, nblocks, s, a)
+                                            : launch2_o<PREC_F64, 2>(wpb, nblocks, s, a);
+    if (prec == PREC_REF32) return order == 1 ? launch2_o<PREC_REF32, 1>(wpb, nblocks, s, a)
+                                              : launch2_o<PREC_REF32, 2>(wpb, nblocks, s, a);
+    if (prec == PREC_C32) return order == 1 ? launch2_o<PREC_C32, 1>(wpb, nblocks, s, a)
+                                            : launch2_o<PREC_C32, 2>(wpb, nblocks, s, a);
+    return hipErrorInvalidValue;
+}
+
+}  // namespace shll

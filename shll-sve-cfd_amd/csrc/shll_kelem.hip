@@ -1,0 +1,279 @@
+// shll_kelem.hip -- per-cell kernels around the step loop.
+//   k_init    : primitives -> state      (Compute_U_from_P, base_shll.c:74-85,
+//                                          base_shll_2d.c:124-137)
+//   k_output  : state -> primitives, conserved (Compute_P_from_U + the arrays
+//                                          Save_Results prints, base_shll.c:161-192)
+//   k_sums    : FP64 conservation sums and a non-finite / non-positive census
+//   k_compare : FP64 L1 / Linf / max-relative differences of two primitive sets
+// Reductions: per-lane sums in a fixed cell order, a wave64 xor-butterfly
+// (__shfl_xor), then the waves of a block in index order; blocks write partials
+// that the host adds in index order, so results are deterministic.
+#include "shll_device.h"
+
+namespace shll {
+
+// primitives (rho, u, (v), T) -> conserved, per precision
+template <int PREC, int DIM>
+__global__ __launch_bounds__(256) void k_init(const Inst* insts, const double* __restrict__ prim, void* out0,
+                                              void* out1, int64_t plane) {
+    const Inst in = insts[blockIdx.y];
+    const int64_t ncell = (int64_t)in.nx * in.ny;
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < ncell;
+         c += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t o = in.off + c;
+        const double r = prim[o], vx = prim[plane + o];
+        const double vy = DIM == 2 ? prim[2 * plane + o] : 0.0;
+        const double T = prim[(DIM == 2 ? 3 : 2) * plane + o];
+        double U[4];
+        if constexpr (PREC == PREC_REF32) {
+            const float rf = (float)r, vxf = (float)vx, vyf = (float)vy, Tf = (float)T, cv = in.k.f_cv;
+            float* u = (float*)out0;
+            u[o] = rf;
+            u[plane + o] = rf * vxf;
+            if constexpr (DIM == 1) {
+                double kin = in.ke == 0 ? (0.5 * (double)vxf) * (double)vxf : 0.5 * (double)(vxf * vxf);
+                u[2 * plane + o] = (float)((double)rf * ((double)(Tf * cv) + kin));
+            } else {
+                u[2 * plane + o] = rf * vyf;
+                u[3 * plane + o] = (float)((double)rf * ((double)(Tf * cv) + 0.5 * (double)(vxf * vxf + vyf * vyf)));
+            }
+            continue;
+        }
+        const double cv = in.k.d_cv;
+        U[0] = r;
+        U[1] = r * vx;
+        if constexpr (DIM == 1) {
+            U[2] = in.ke == 0 ? r * (T * cv + (0.5 * vx) * vx) : r * (T * cv + 0.5 * (vx * vx));
+        } else {
+            U[2] = r * vy;
+            U[3] = r * (T * cv + 0.5 * (vx * vx + vy * vy));
+        }
+        if constexpr (PREC == PREC_F64) {
+            double* u = (double*)out0;
+#pragma unroll
+            for (int v = 0; v < DIM + 2; ++v) u[v * plane + o] = U[v];
+        } else {
+            float* h = (float*)out0;
+            float* l = (float*)out1;
+#pragma unroll
+            for (int v = 0; v < DIM + 2; ++v) {
+                float hv = (float)U[v];
+                h[v * plane + o] = hv;
+                l[v * plane + o] = (float)(U[v] - (double)hv);
+            }
+        }
+    }
+}
+
+// conserved values of one cell as doubles
+template <int PREC, int DIM>
+__device__ __forceinline__ void load_cons(const void* s0, const void* s1, int64_t plane, int64_t o, double (&U)[4]) {
+#pragma unroll
+    for (int v = 0; v < DIM + 2; ++v) {
+        if constexpr (PREC == PREC_F64)
+            U[v] = ((const double*)s0)[v * plane + o];
+        else if constexpr (PREC == PREC_REF32)
+            U[v] = (double)((const float*)s0)[v * plane + o];
+        else
+            U[v] = (double)((const float*)s0)[v * plane + o] + (double)((const float*)s1)[v * plane + o];
+    }
+}
+
+template <int PREC, int DIM>
+__global__ __launch_bounds__(256) void k_output(const Inst* insts, const void* s0, const void* s1, int64_t plane,
+                                                double* __restrict__ prim, double* __restrict__ cons) {
+    const Inst in = insts[blockIdx.y];
+    const int64_t ncell = (int64_t)in.nx * in.ny;
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < ncell;
+         c += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t o = in.off + c;
+        double U[4] = {0, 0, 0, 0}, p[4] = {0, 0, 0, 0};
+        load_cons<PREC, DIM>(s0, s1, plane, o, U);
+        if constexpr (PREC == PREC_REF32) {
+            const float* u = (const float*)s0;
+            float vx, vy = 0.f, T, a;
+            if constexpr (DIM == 1) {
+                RefArith<float>::prim1(u[o], u[plane + o], u[2 * plane + o], in.ke, in.k.f_cv, in.k.f_gR, vx, T, a);
+                p[0] = u[o]; p[1] = vx; p[2] = T;
+            } else {
+                RefArith<float>::prim2(u[o], u[plane + o], u[2 * plane + o], u[3 * plane + o], in.k.f_cv, in.k.f_gR,
+                                       vx, vy, T, a);
+                p[0] = u[o]; p[1] = vx; p[2] = vy; p[3] = T;
+            }
+        } else {
+            const double cv = in.k.d_cv, r = U[0], vx = U[1] / r;
+            p[0] = r;
+            p[1] = vx;
+            if constexpr (DIM == 1) {
+                p[2] = in.ke == 0 ? ((U[2] / r) - (0.5 * vx) * vx) / cv : ((U[2] / r) - 0.5 * (vx * vx)) / cv;
+            } else {
+                const double vy = U[2] / r;
+                p[2] = vy;
+                p[3] = ((U[3] / r) - 0.5 * (vx * vx + vy * vy)) / cv;
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < DIM + 2; ++v) {
+            if (prim) prim[v * plane + o] = p[v];
+            if (cons) cons[v * plane + o] = U[v];
+        }
+    }
+}
+
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) x += __shfl_xor(x, d, 64);
+    return x;
+}
+__device__ __forceinline__ double wave_max(double x) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) x = fmax(x, __shfl_xor(x, d, 64));
+    return x;
+}
+
+// Block reduction of NS sums then NM maxima; thread 0 writes them to out.
+template <int NS, int NM>
+__device__ __forceinline__ void block_reduce(double (&s)[NS], double (&mx)[NM], double* out) {
+    __shared__ double red[4][NS + NM];
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+#pragma unroll
+    for (int q = 0; q < NS; ++q) s[q] = wave_sum(s[q]);
+#pragma unroll
+    for (int q = 0; q < NM; ++q) mx[q] = wave_max(mx[q]);
+    if (l == 0) {
+#pragma unroll
+        for (int q = 0; q < NS; ++q) red[w][q] = s[q];
+#pragma unroll
+        for (int q = 0; q < NM; ++q) red[w][NS + q] = mx[q];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = blockDim.x >> 6;
+        for (int q = 0; q < NS; ++q) {
+            double t = 0;
+            for (int k = 0; k < nw; ++k) t += red[k][q];
+            out[q] = t;
+        }
+        for (int q = 0; q < NM; ++q) {
+            double t = red[0][NS + q];
+            for (int k = 1; k < nw; ++k) t = fmax(t, red[k][NS + q]);
+            out[NS + q] = t;
+        }
+    }
+}
+
+// partial[(inst * gridDim.x + block) * 5 + q]: sums of the DIM+2 conserved variables
+// (unused slots 0) and, at q = 4, the count of cells whose density or energy is
+// non-finite or not positive.
+template <int PREC, int DIM>
+__global__ __launch_bounds__(256) void k_sums(const Inst* insts, const void* s0, const void* s1, int64_t plane,
+                                              double* partial) {
+    const Inst in = insts[blockIdx.y];
+    const int64_t ncell = (int64_t)in.nx * in.ny;
+    double s[5] = {0, 0, 0, 0, 0};
+    double none[1] = {0};
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < ncell;
+         c += (int64_t)gridDim.x * blockDim.x) {
+        double U[4] = {0, 0, 0, 0};
+        load_cons<PREC, DIM>(s0, s1, plane, in.off + c, U);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) s[v] += U[v];
+        const double E = U[DIM + 1];
+        const bool good = U[0] > 0.0 && E > 0.0 && isfinite(U[0]) && isfinite(U[1]) && isfinite(E) &&
+                          (DIM == 1 || isfinite(U[2]));
+        s[4] += good ? 0.0 : 1.0;
+    }
+    block_reduce<5, 1>(s, none, partial + ((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * 6);
+}
+
+// Differences of two primitive sets (nv planes, ncell cells from offset off, plane stride):
+// sums |drho|, |du|, |dp| (p = rho*T, R = 1), maxima |drho|/rho_b, |dp|/p_b, |du|, |u_b|,
+// |drho|, |dp|, and a count of non-finite entries.
+__global__ __launch_bounds__(256) void k_compare(const double* __restrict__ a, const double* __restrict__ b,
+                                                 int64_t plane, int64_t off, int64_t ncell, int dim,
+                                                 double* partial) {
+    double s[4] = {0, 0, 0, 0};
+    double mx[6] = {0, 0, 0, 0, 0, 0};
+    const int tv = dim == 2 ? 3 : 2;
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < ncell;
+         c += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t o = off + c;
+        const double ra = a[o], rb = b[o], ua = a[plane + o], ub = b[plane + o];
+        const double pa = ra * a[tv * plane + o], pb = rb * b[tv * plane + o];
+        const double dr = fabs(ra - rb), du = fabs(ua - ub), dp = fabs(pa - pb);
+        double dv = 0, vb = 0;
+        if (dim == 2) {
+            dv = fabs(a[2 * plane + o] - b[2 * plane + o]);
+            vb = fabs(b[2 * plane + o]);
+        }
+        const bool fin = isfinite(ra) && isfinite(ua) && isfinite(pa) && isfinite(dv);
+        s[0] += dr;
+        s[1] += du;
+        s[2] += dp;
+        s[3] += fin ? 0.0 : 1.0;
+        mx[0] = fmax(mx[0], dr / fabs(rb));
+        mx[1] = fmax(mx[1], dp / fabs(pb));
+        mx[2] = fmax(mx[2], fmax(du, dv));
+        mx[3] = fmax(mx[3], fmax(fabs(ub), vb));
+        mx[4] = fmax(mx[4], dr);
+        mx[5] = fmax(mx[5], dp);
+    }
+    block_reduce<4, 6>(s, mx, partial + (int64_t)blockIdx.x * 10);
+}
+
+// ---------------------------------------------------------------------------
+template <int PREC>
+static hipError_t init_p(int dim, dim3 g, hipStream_t st, const Inst* insts, const double* prim, void* o0,
+                         void* o1, int64_t plane) {
+    if (dim == 1) hipLaunchKernelGGL((k_init<PREC, 1>), g, dim3(256), 0, st, insts, prim, o0, o1, plane);
+    else hipLaunchKernelGGL((k_init<PREC, 2>), g, dim3(256), 0, st, insts, prim, o0, o1, plane);
+    return hipGetLastError();
+}
+
+hipError_t launch_init(int prec, int dim, int nblocks, int n_inst, hipStream_t st, const Inst* insts,
+                       const double* prim, void* o0, void* o1, int64_t plane) {
+    dim3 g(nblocks, n_inst);
+    if (prec == PREC_F64) return init_p<PREC_F64>(dim, g, st, insts, prim, o0, o1, plane);
+    if (prec == PREC_REF32) return init_p<PREC_REF32>(dim, g, st, insts, prim, o0, o1, plane);
+    return init_p<PREC_C32>(dim, g, st, insts, prim, o0, o1, plane);
+}
+
+template <int PREC>
+static hipError_t output_p(int dim, dim3 g, hipStream_t st, const Inst* insts, const void* s0, const void* s1,
+                           int64_t plane, double* prim, double* cons) {
+    if (dim == 1) hipLaunchKernelGGL((k_output<PREC, 1>), g, dim3(256), 0, st, insts, s0, s1, plane, prim, cons);
+    else hipLaunchKernelGGL((k_output<PREC, 2>), g, dim3(256), 0, st, insts, s0, s1, plane, prim, cons);
+    return hipGetLastError();
+}
+
+hipError_t launch_output(int prec, int dim, int nblocks, int n_inst, hipStream_t st, const Inst* insts,
+                         const void* s0, const void* s1, int64_t plane, double* prim, double* cons) {
+    dim3 g(nblocks, n_inst);
+    if (prec == PREC_F64) return output_p<PREC_F64>(dim, g, st, insts, s0, s1, plane, prim, cons);
+    if (prec == PREC_REF32) return output_p<PREC_REF32>(dim, g, st, insts, s0, s1, plane, prim, cons);
+    return output_p<PREC_C32>(dim, g, st, insts, s0, s1, plane, prim, cons);
+}
+
+template <int PREC>
+static hipError_t sums_p(int dim, dim3 g, hipStream_t st, const Inst* insts, const void* s0, const void* s1,
+                         int64_t plane, double* partial) {
+    if (dim == 1) hipLaunchKernelGGL((k_sums<PREC, 1>), g, dim3(256), 0, st, insts, s0, s1, plane, partial);
+    else hipLaunchKernelGGL((k_sums<PREC, 2>), g, dim3(256), 0, st, insts, s0, s1, plane, partial);
+    return hipGetLastError();
+}
+
+hipError_t launch_sums(int prec, int dim, int nblocks, int n_inst, hipStream_t st, const Inst* insts,
+                       const void* s0, const void* s1, int64_t plane, double* partial) {
+    dim3 g(nblocks, n_inst);
+    if (prec == PREC_F64) return sums_p<PREC_F64>(dim, g, st, insts, s0, s1, plane, partial);
+    if (prec == PREC_REF32) return sums_p<PREC_REF32>(dim, g, st, insts, s0, s1, plane, partial);
+    return sums_p<PREC_C32>(dim, g, st, insts, s0, s1, plane, partial);
+}
+
+hipError_t launch_compare(int nblocks, hipStream_t st, const double* a, const double* b, int64_t plane, int64_t off,
+                          int64_t ncell, int dim, double* partial) {
+    hipLaunchKernelGGL(k_compare, dim3(nblocks), dim3(256), 0, st, a, b, plane, off, ncell, dim, partial);
+    return hipGetLastError();
+}
+
+}  // namespace shll

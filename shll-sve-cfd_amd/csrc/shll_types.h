@@ -1,0 +1,66 @@
+// shll_types.h -- tables and launch arguments shared by the host API and the kernels.
+#pragma once
+
+#include <stdint.h>
+
+namespace shll {
+
+enum Prec : int { PREC_F64 = 0, PREC_REF32 = 1, PREC_C32 = 2 };
+enum Bc : int { BC_REFLECT = 0, BC_OUTFLOW = 1 };
+
+// Per-instance scheme constants, computed on the host (never folded on the device):
+//   F64   : d_*  (gamma = 1.4, CV = R/(gamma-1), lam = DT/DX)
+//   REF32 : f_*  (the reference's float constants, base_shll.c:17-22,83-84)
+//   C32   : c_*  (the F64 constants rounded once to float; rcv = 1/CV)
+struct Consts {
+    double d_cv, d_gR, d_R, d_lamx, d_lamy;
+    float f_cv, f_gR, f_R, f_lamx, f_lamy;
+    float c_rcv, c_gR, c_R, c_lamx, c_lamy;
+    float pad0;
+};
+
+// One instance (tube or 2D domain) of a launch.
+struct Inst {
+    int64_t off;     // first cell of the instance in every variable plane
+    int64_t nsteps;  // steps this instance takes in total (batch); huge for single runs
+    int32_t nx;      // cells in x
+    int32_t ny;      // cells in y (1 in 1D)
+    int32_t bc;      // Bc
+    int32_t ke;      // kinetic-energy association of the reference file (0: base_shll.c, 1: 2D files)
+    Consts k;
+};
+
+// One wavefront's share of a 1D launch: it loads cells [g0, g0 + 64*C) of its
+// instance (cells outside [0, nx) get a benign state), advances them, and writes
+// back the owned cells [x0, x0 + w), all of which are still exact after the
+// launch's steps (g0 <= x0 - order*steps or g0 <= 0, and the same on the right).
+struct Item1 {
+    int32_t inst;
+    int32_t g0;
+    int32_t x0;
+    int32_t w;
+};
+
+// One wavefront's share of a 2D launch: a strip of columns j in [j0, j0 + 64)
+// (owned [jx0, jx0 + jw)) streamed along rows i in [i0, i0 + ilen) (owned
+// [ix0, ix0 + iw)).
+struct Item2 {
+    int32_t inst;
+    int32_t j0, jx0, jw;
+    int32_t i0, ix0, iw, ilen;
+};
+
+struct StepArgs {
+    const Inst* inst;
+    const void* items;
+    int32_t n_items;
+    int32_t ksteps;      // steps of this launch (an instance does min(ksteps, nsteps - step_base))
+    int64_t step_base;   // steps already taken before this launch
+    int64_t plane;       // cells per variable plane
+    const void* in0;     // state read: F64 double*, REF32 float*, C32 hi float*
+    const void* in1;     //             C32 lo float*
+    void* out0;          // state written
+    void* out1;
+};
+
+}  // namespace shll

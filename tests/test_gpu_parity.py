@@ -1,0 +1,219 @@
+"""GPU parity: the HIP kernels (through the C ABI) against the CPU oracle on the same
+inputs.  F64 and REF32 are compared bitwise (the oracle's reference association and
+IEEE rounding are reproduced exactly); C32 is compared bitwise against the oracle's
+restatement of the compensated scheme and by its distance to F64."""
+import os
+
+import numpy as np
+import pytest
+
+import icgen
+import shll_amd as S
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+MODES = {S.F64: 0, S.REF32: 1, S.C32: 2}  # library precision -> oracle mode
+
+
+def banner(*a):
+    print("[gpu-case]", *a, flush=True)
+
+
+def opb(orc, pb):
+    return orc.problem(pb.dim, pb.order, pb.nx, pb.ny, bc=pb.bc, lx=pb.lx, ly=pb.ly)
+
+
+def gpu_run(pb, prim, nsteps, launch=None):
+    with S.Solver(pb, prim) as s:
+        if launch:
+            s.set_launch(*launch)
+        s.step(nsteps)
+        raw = s.state_raw()
+        return s.primitives(), s.conserved(), raw
+
+
+def oracle_run(orc, pb, prim, nsteps):
+    o = opb(orc, pb)
+    mode = MODES[pb.precision]
+    p, c = orc.run(o, mode, prim, nsteps, nthreads=4)
+    raw = orc.run_c32_raw(o, prim, nsteps) if pb.precision == S.C32 else None
+    return p, c, raw
+
+
+def assert_same(pb, gpu, ora, what):
+    gp, gc, graw = gpu
+    op, oc, oraw = ora
+    if pb.precision == S.C32:
+        np.testing.assert_array_equal(graw[0], oraw[0], err_msg=what + " hi")
+        np.testing.assert_array_equal(graw[1], oraw[1], err_msg=what + " lo")
+    np.testing.assert_array_equal(gc, oc, err_msg=what + " conserved")
+    np.testing.assert_array_equal(gp, op, err_msg=what + " primitives")
+
+
+@pytest.mark.parametrize("prec", [S.F64, S.REF32, S.C32])
+@pytest.mark.parametrize("order", [1, 2])
+def test_1d_sod_full_run(orc, prec, order):
+    """configs[0] shape: Sod, 1024 cells to t = 0.2 (1639 steps)."""
+    pb = S.problem(1, order, 1024, precision=prec)
+    n = S.steps_to_time(pb, 0.2)
+    assert n == 1639
+    ic = S.ic_tube(1024)
+    banner("1d sod", prec, order)
+    assert_same(pb, gpu_run(pb, ic, n), oracle_run(orc, pb, ic, n), "sod p%d o%d" % (prec, order))
+
+
+@pytest.mark.parametrize("launch", [(1, 1, 1, 0), (1, 8, 4, 64), (2, 3, 1, 0), (4, 16, 4, 128), (8, 5, 4, 20),
+                                    (2, 24, 1, 0), (8, 1, 1, 8), (8, 60, 1, 0)])
+@pytest.mark.parametrize("prec", [S.F64, S.C32])
+def test_1d_launch_shapes_are_bitwise_identical(orc, launch, prec):
+    """Every (cells/lane, steps/launch, waves/block, graph) gives the oracle's bits."""
+    n = 3001
+    pb = S.problem(1, 2, n, precision=prec)
+    ic = icgen.random_1d(n, 5, block=37)
+    banner("1d shapes", launch, prec)
+    assert_same(pb, gpu_run(pb, ic, 150, launch), oracle_run(orc, pb, ic, 150), "shape %s" % (launch,))
+
+
+def test_1d_impossible_launch_shape_is_rejected():
+    pb = S.problem(1, 2, 3001)
+    with S.Solver(pb, S.ic_tube(3001)) as s:
+        before = s.get_launch()
+        with pytest.raises(S.ShllError) as e:
+            s.set_launch(2, 64, 1, 0)  # 128-cell segments, 128-cell halo each side
+        assert e.value.code == S.EINVAL
+        assert s.get_launch() == before
+        s.step(3)
+
+
+@pytest.mark.parametrize("order,bc", [(1, 0), (1, 1), (2, 0), (2, 1)])
+@pytest.mark.parametrize("prec", [S.F64, S.REF32, S.C32])
+def test_1d_walls_random_states(orc, order, bc, prec):
+    n = 700
+    pb = S.problem(1, order, n, precision=prec, bc=bc)
+    ic = icgen.random_1d(n, 40 + order, block=1)
+    banner("1d walls", order, bc, prec)
+    assert_same(pb, gpu_run(pb, ic, 3, (2, 2, 1, 0)), oracle_run(orc, pb, ic, 3), "walls")
+    ic = icgen.random_1d(n, 50 + order, block=25)
+    assert_same(pb, gpu_run(pb, ic, 60, (4, 7, 4, 0)), oracle_run(orc, pb, ic, 60), "walls blocky")
+
+
+def test_1d_reference_golden_and_results_file(tmp_path):
+    """REF32 on the GPU reproduces the reference's own run (base_shll.c, N=256, 410 steps)
+    bitwise, and its results.dat byte for byte."""
+    g = np.load(os.path.join(GOLD, "o1_1d_main.npz"), allow_pickle=False)
+    pb = S.problem(1, 1, 256, precision=S.REF32)
+    banner("1d golden")
+    with S.Solver(pb, S.ic_tube(256)) as s:
+        assert s.run_to_time(0.2) == 410
+        p = s.primitives().astype(np.float32)
+        u = s.state_raw()
+        np.testing.assert_array_equal(p, g["p_410"])
+        np.testing.assert_array_equal(u, g["u_410"])
+        s.write_results(tmp_path / "results.dat")
+        assert (tmp_path / "results.dat").read_bytes() == bytes(g["results_dat"])
+
+
+@pytest.mark.parametrize("prec", [S.F64, S.REF32, S.C32])
+@pytest.mark.parametrize("order", [1, 2])
+def test_2d_random_and_circle(orc, prec, order):
+    banner("2d", prec, order)
+    pb = S.problem(2, order, 70, 150, precision=prec)
+    ic = icgen.random_2d(70, 150, 9, block=1)
+    assert_same(pb, gpu_run(pb, ic, 2), oracle_run(orc, pb, ic, 2), "2d rough")
+    ic = icgen.random_2d(70, 150, 10, block=13)
+    assert_same(pb, gpu_run(pb, ic, 25), oracle_run(orc, pb, ic, 25), "2d blocky")
+    pb = S.problem(2, order, 96, 96, precision=prec)
+    ic = S.ic_circle(96, 96)
+    assert_same(pb, gpu_run(pb, ic, 60), oracle_run(orc, pb, ic, 60), "2d circle")
+
+
+@pytest.mark.parametrize("name,order,steps", [("o1_2d_main", 1, 205), ("o2_2d_fourshocks", 2, 50)])
+def test_2d_reference_golden(name, order, steps):
+    """REF32 on the GPU reproduces the reference's 2D solvers at 256^2 bitwise."""
+    import hashlib
+    g = np.load(os.path.join(GOLD, name + ".npz"), allow_pickle=False)
+    pb = S.problem(2, order, 256, 256, precision=S.REF32)
+    ic = S.ic_implosion(256, 256) if order == 1 else S.ic_four_shocks(256, 256)
+    banner("2d golden", name)
+    with S.Solver(pb, ic) as s:
+        s.step(steps)
+        p = s.primitives().astype(np.float32)
+        u = s.state_raw()
+    dig = [hashlib.sha256(np.ascontiguousarray(x).tobytes()).hexdigest() for x in list(p) + list(u)]
+    assert dig == list(g["dig_%d" % steps][:8])
+
+
+def test_2d_launch_shapes(orc):
+    pb = S.problem(2, 2, 200, 130, precision=S.F64)
+    ic = icgen.random_2d(200, 130, 3, block=9)
+    ora = oracle_run(orc, pb, ic, 12)
+    for launch in [(1, 1, 1, 0), (1, 1, 4, 4), (1, 1, 4, 0)]:
+        banner("2d shapes", launch)
+        assert_same(pb, gpu_run(pb, ic, 12, launch), ora, "2d shape %s" % (launch,))
+
+
+@pytest.mark.parametrize("prec", [S.F64, S.C32])
+@pytest.mark.parametrize("order", [1, 2])
+def test_1d_batch_equals_single_runs(orc, prec, order):
+    """A batch of uneven tubes: every instance equals its own oracle run, and finished
+    instances stay frozen while longer ones continue."""
+    pbs, prims, steps = S.tube_batch(777, 12, nmin=64, nmax=3000, precision=prec, order=order)
+    steps = [max(1, s // 8) for s in steps]
+    b = S.Batch(pbs, prims, steps)
+    b.set_launch(2, 8, 4, 32)
+    banner("1d batch", prec, order)
+    b.run()
+    for k in range(len(pbs)):
+        ref = oracle_run(orc, pbs[k], prims[k], steps[k])
+        np.testing.assert_array_equal(b.conserved(k), ref[1], err_msg="instance %d" % k)
+        np.testing.assert_array_equal(b.primitives(k), ref[0], err_msg="instance %d" % k)
+    b.close()
+
+
+def test_2d_batch_equals_single_runs(orc):
+    pbs, prims = S.domain_batch(31, 4, smin=40, smax=120, precision=S.C32)
+    steps = [30, 11, 20, 5]
+    b = S.Batch(pbs, prims, steps)
+    banner("2d batch")
+    b.run()
+    for k in range(len(pbs)):
+        ref = oracle_run(orc, pbs[k], prims[k], steps[k])
+        np.testing.assert_array_equal(b.conserved(k), ref[1], err_msg="domain %d" % k)
+    b.close()
+
+
+@pytest.mark.parametrize("prec", [S.F64, S.C32])
+def test_conservation_reflective(prec):
+    """Reflective walls: mass and energy are conserved (F64 to round-off; C32's face
+    update telescopes, so to ~1e-14)."""
+    pb = S.problem(1, 1, 4096, precision=prec)
+    banner("conservation", prec)
+    with S.Solver(pb, S.ic_tube(4096)) as s:
+        s0 = s.conserved_sums()
+        s.step(3000)
+        s1 = s.conserved_sums()
+        assert s.check_state() == 0
+    assert abs(s1[0] - s0[0]) <= 1e-13 * abs(s0[0])
+    assert abs(s1[2] - s0[2]) <= 1e-13 * abs(s0[2])
+    pb = S.problem(2, 1, 128, 128, precision=prec)
+    with S.Solver(pb, S.ic_circle(128, 128)) as s:
+        s0 = s.conserved_sums()
+        s.step(300)
+        s1 = s.conserved_sums()
+    assert abs(s1[0] - s0[0]) <= 1e-13 * abs(s0[0])
+    assert abs(s1[3] - s0[3]) <= 1e-13 * abs(s0[3])
+
+
+def test_c32_accuracy_against_f64_on_gpu():
+    """FP32-compensated first-order Sod (configs[0] shape) vs the FP64 kernel: L1(rho) <= 1e-6."""
+    pb64 = S.problem(1, 1, 1024, precision=S.F64)
+    pb32 = S.problem(1, 1, 1024, precision=S.C32)
+    ic = S.ic_tube(1024)
+    banner("c32 accuracy")
+    with S.Solver(pb64, ic) as a, S.Solver(pb32, ic) as b:
+        a.run_to_time(0.2)
+        b.run_to_time(0.2)
+        e = b.compare(a)
+    assert e["l1_rho"] <= 1e-6, e
+    assert e["nonfinite"] == 0
