@@ -160,8 +160,14 @@ shll_status shll_batch_get_primitives(shll_batch *b, int32_t k, double *out_soa)
 shll_status shll_batch_get_conserved(shll_batch *b, int32_t k, double *out_soa);
 /* FP64 conserved sums per instance: out[k*4 + v] (v < nvar), cell volume included. */
 shll_status shll_batch_conserved_sums(shll_batch *b, double *out);
+/* Per-instance census of non-finite / non-positive density or energy: bad_cells[k]
+ * (may be NULL) and the number of instances with any (may be NULL). */
+shll_status shll_batch_check_state(shll_batch *b, int64_t *bad_cells, int32_t *bad_instances);
+/* Error norms of instance k of b against instance kref of ref (same shape), FP64 on the GPU. */
+shll_status shll_batch_compare(shll_batch *b, int32_t k, shll_batch *ref, int32_t kref, shll_errors *out);
 shll_status shll_batch_set_launch(shll_batch *b, int32_t cells_per_lane, int32_t steps_per_launch,
                                   int32_t waves_per_block, int32_t graph_steps);
+shll_status shll_batch_get_launch(const shll_batch *b, int32_t out[4]);
 shll_status shll_batch_autotune(shll_batch *b, int32_t trial_steps);
 shll_status shll_batch_set_timing(shll_batch *b, int32_t on);
 shll_status shll_batch_last_timing(const shll_batch *b, double *avg_kernel_ms, int64_t *launches, double *total_ms);

@@ -31,8 +31,8 @@ hipError_t launch_output(int prec, int dim, int nblocks, int n_inst, hipStream_t
                          const void* s0, const void* s1, int64_t plane, double* prim, double* cons);
 hipError_t launch_sums(int prec, int dim, int nblocks, int n_inst, hipStream_t st, const Inst* insts,
                        const void* s0, const void* s1, int64_t plane, double* partial);
-hipError_t launch_compare(int nblocks, hipStream_t st, const double* a, const double* b, int64_t plane,
-                          int64_t off, int64_t ncell, int dim, double* partial);
+hipError_t launch_compare(int nblocks, hipStream_t st, const double* a, int64_t plane_a, const double* b,
+                          int64_t plane_b, int64_t ncell, int dim, double* partial);
 }  // namespace shll
 
 using namespace shll;
@@ -235,6 +235,10 @@ struct Engine {
     std::vector<hipGraphNode_t> bnodes;
     int b_launches = 0;
     int b_key[3] = {0, 0, 0};
+    // kernel parameters of the batch graph nodes: kept alive for the life of the graph
+    // (hipKernelNodeParams.kernelParams points into these)
+    std::vector<StepArgs> b_args;
+    std::vector<void*> b_kp;
     // diagnostics scratch
     double* d_prim = nullptr;
     double* d_cons = nullptr;
@@ -432,34 +436,33 @@ static shll_status batch_launches(Engine* E, int64_t nlaunch) {
         if (g > 0 && jend - j >= g) {
             const int key[3] = {E->L.C, E->L.k, g};
             if (E->bexec && std::memcmp(key, E->b_key, sizeof(key)) != 0) engine_drop_graphs(E);
-            std::vector<StepArgs> args(g);
+            if ((int)E->b_args.size() != g) {
+                E->b_args.assign(g, StepArgs{});
+                E->b_kp.assign(g, nullptr);
+            }
             std::vector<hipKernelNodeParams> prm(g);
             for (int q = 0; q < g; ++q) {
                 const int64_t na = active_items(E, j + q);
-                args[q] = make_args(E, j + q, (int)((j + q) & 1), na);
+                E->b_args[q] = make_args(E, j + q, (int)((j + q) & 1), na);
+                E->b_kp[q] = &E->b_args[q];
                 std::memset(&prm[q], 0, sizeof(prm[q]));
                 prm[q].func = (void*)kernel_fn(E);
                 prm[q].gridDim = dim3((unsigned)std::max<int64_t>(1, (na + wpb - 1) / wpb));
                 prm[q].blockDim = dim3(64 * wpb);
                 prm[q].sharedMemBytes = 0;
+                prm[q].kernelParams = &E->b_kp[q];
             }
             if (!E->bexec) {
                 HIPCHECK(hipGraphCreate(&E->bgraph, 0));
                 E->bnodes.assign(g, nullptr);
-                for (int q = 0; q < g; ++q) {
-                    void* kp[1] = {&args[q]};
-                    prm[q].kernelParams = kp;
+                for (int q = 0; q < g; ++q)
                     HIPCHECK(hipGraphAddKernelNode(&E->bnodes[q], E->bgraph, q ? &E->bnodes[q - 1] : nullptr,
                                                    q ? 1 : 0, &prm[q]));
-                }
                 HIPCHECK(hipGraphInstantiate(&E->bexec, E->bgraph, nullptr, nullptr, 0));
                 std::memcpy(E->b_key, key, sizeof(key));
             } else {
-                for (int q = 0; q < g; ++q) {
-                    void* kp[1] = {&args[q]};
-                    prm[q].kernelParams = kp;
+                for (int q = 0; q < g; ++q)
                     HIPCHECK(hipGraphExecKernelNodeSetParams(E->bexec, E->bnodes[q], &prm[q]));
-                }
             }
             HIPCHECK(hipGraphLaunch(E->bexec, E->stream));
             j += g;
@@ -812,12 +815,23 @@ extern "C" shll_status shll_check_state(shll_ctx* ctx, int64_t* bad_cells) {
     return SHLL_OK;
 }
 
-static shll_status compare_dev(Engine* E, int dim, const double* a, const double* b, int64_t plane, int64_t ncell,
-                               double vol, shll_errors* out) {
+// primitives of instance i on the device (E->d_prim at the instance's offset)
+static shll_status engine_prim_dev(Engine* E, int64_t i, const double** out) {
+    if (!E->d_prim) HIPCHECK(hipMalloc(&E->d_prim, (size_t)E->plane * E->nv * sizeof(double)));
+    const int p = inst_parity(E, i);
+    HIPCHECK(launch_output(E->prec, E->dim, elem_blocks(E->inst_cells[i]), 1, E->stream, E->d_inst + i,
+                           E->buf[p][0], E->buf[p][1], E->plane, E->d_prim, nullptr));
+    HIPCHECK(hipStreamSynchronize(E->stream));
+    *out = E->d_prim + E->h_inst[i].off;
+    return SHLL_OK;
+}
+
+static shll_status compare_dev(Engine* E, int dim, const double* a, int64_t plane_a, const double* b,
+                               int64_t plane_b, int64_t ncell, double vol, shll_errors* out) {
     const int nb = std::min(elem_blocks(ncell), 512);
     shll_status st = ensure_partial(E, (size_t)nb * 10);
     if (st != SHLL_OK) return st;
-    HIPCHECK(launch_compare(nb, E->stream, a, b, plane, 0, ncell, dim, E->d_partial));
+    HIPCHECK(launch_compare(nb, E->stream, a, plane_a, b, plane_b, ncell, dim, E->d_partial));
     std::vector<double> part((size_t)nb * 10);
     HIPCHECK(hipMemcpyAsync(part.data(), E->d_partial, part.size() * sizeof(double), hipMemcpyDeviceToHost, E->stream));
     HIPCHECK(hipStreamSynchronize(E->stream));
@@ -846,14 +860,11 @@ extern "C" shll_status shll_compare(shll_ctx* ctx, shll_ctx* ref, shll_errors* o
     if (ctx->E->device != ref->E->device) return fail(SHLL_EINVAL, "contexts on different devices");
     HIPCHECK(hipSetDevice(ctx->E->device));
     Engine *A = ctx->E, *B = ref->E;
-    const int64_t nc = A->inst_cells[0];
-    for (Engine* E : {A, B}) {
-        if (!E->d_prim) HIPCHECK(hipMalloc(&E->d_prim, (size_t)E->plane * E->nv * sizeof(double)));
-        HIPCHECK(launch_output(E->prec, E->dim, elem_blocks(nc), 1, E->stream, E->d_inst, E->buf[E->cur][0],
-                               E->buf[E->cur][1], E->plane, E->d_prim, nullptr));
-        HIPCHECK(hipStreamSynchronize(E->stream));
-    }
-    return compare_dev(A, A->dim, A->d_prim, B->d_prim, A->plane, nc, cell_volume(ctx->pb), out);
+    const double *pa = nullptr, *pb = nullptr;
+    shll_status st = engine_prim_dev(A, 0, &pa);
+    if (st == SHLL_OK) st = engine_prim_dev(B, 0, &pb);
+    if (st != SHLL_OK) return st;
+    return compare_dev(A, A->dim, pa, A->plane, pb, B->plane, A->inst_cells[0], cell_volume(ctx->pb), out);
 }
 
 extern "C" shll_status shll_compare_host(shll_ctx* ctx, const double* ref_prim_soa, shll_errors* out) {
@@ -861,15 +872,15 @@ extern "C" shll_status shll_compare_host(shll_ctx* ctx, const double* ref_prim_s
     Engine* E = ctx->E;
     HIPCHECK(hipSetDevice(E->device));
     const int64_t nc = E->inst_cells[0];
-    if (!E->d_prim) HIPCHECK(hipMalloc(&E->d_prim, (size_t)E->plane * E->nv * sizeof(double)));
+    const double* pa = nullptr;
+    shll_status st = engine_prim_dev(E, 0, &pa);
+    if (st != SHLL_OK) return st;
     if (!E->d_cons) HIPCHECK(hipMalloc(&E->d_cons, (size_t)E->plane * E->nv * sizeof(double)));
-    HIPCHECK(launch_output(E->prec, E->dim, elem_blocks(nc), 1, E->stream, E->d_inst, E->buf[E->cur][0],
-                           E->buf[E->cur][1], E->plane, E->d_prim, nullptr));
     for (int v = 0; v < E->nv; ++v)
         HIPCHECK(hipMemcpyAsync(E->d_cons + (size_t)v * E->plane, ref_prim_soa + (size_t)v * nc, nc * sizeof(double),
                                 hipMemcpyHostToDevice, E->stream));
     HIPCHECK(hipStreamSynchronize(E->stream));
-    return compare_dev(E, E->dim, E->d_prim, E->d_cons, E->plane, nc, cell_volume(ctx->pb), out);
+    return compare_dev(E, E->dim, pa, E->plane, E->d_cons, E->plane, nc, cell_volume(ctx->pb), out);
 }
 
 static shll_status set_launch(Engine* E, int32_t C, int32_t k, int32_t wpb, int32_t graph) {
@@ -1169,10 +1180,50 @@ extern "C" shll_status shll_batch_conserved_sums(shll_batch* b, double* out) {
     return SHLL_OK;
 }
 
+extern "C" shll_status shll_batch_check_state(shll_batch* b, int64_t* bad_cells, int32_t* bad_instances) {
+    if (!b) return fail(SHLL_EINVAL, "batch is NULL");
+    HIPCHECK(hipSetDevice(b->E->device));
+    std::vector<double> s;
+    shll_status st = engine_sums(b->E, s);
+    if (st != SHLL_OK) return st;
+    int32_t nbad = 0;
+    for (size_t i = 0; i < b->pbs.size(); ++i) {
+        const int64_t bad = (int64_t)s[i * 5 + 4];
+        if (bad_cells) bad_cells[i] = bad;
+        nbad += bad > 0;
+    }
+    if (bad_instances) *bad_instances = nbad;
+    return SHLL_OK;
+}
+
+extern "C" shll_status shll_batch_compare(shll_batch* b, int32_t k, shll_batch* ref, int32_t kref,
+                                          shll_errors* out) {
+    if (!b || !ref || !out || k < 0 || k >= (int32_t)b->pbs.size() || kref < 0 || kref >= (int32_t)ref->pbs.size())
+        return fail(SHLL_EINVAL, "bad argument");
+    const shll_problem &pa = b->pbs[k], &pr = ref->pbs[kref];
+    if (pa.dim != pr.dim || pa.nx != pr.nx || pa.ny != pr.ny) return fail(SHLL_EINVAL, "instances differ in shape");
+    if (b->E->device != ref->E->device) return fail(SHLL_EINVAL, "batches on different devices");
+    HIPCHECK(hipSetDevice(b->E->device));
+    const double *x = nullptr, *y = nullptr;
+    shll_status st = engine_prim_dev(b->E, k, &x);
+    if (st == SHLL_OK) st = engine_prim_dev(ref->E, kref, &y);
+    if (st != SHLL_OK) return st;
+    return compare_dev(b->E, pa.dim, x, b->E->plane, y, ref->E->plane, b->E->inst_cells[k], cell_volume(pa), out);
+}
+
 extern "C" shll_status shll_batch_set_launch(shll_batch* b, int32_t C, int32_t k, int32_t wpb, int32_t graph) {
     if (!b) return fail(SHLL_EINVAL, "batch is NULL");
     HIPCHECK(hipSetDevice(b->E->device));
     return set_launch(b->E, C, k, wpb, graph);
+}
+
+extern "C" shll_status shll_batch_get_launch(const shll_batch* b, int32_t out[4]) {
+    if (!b || !out) return fail(SHLL_EINVAL, "NULL argument");
+    out[0] = b->E->L.C;
+    out[1] = b->E->L.k;
+    out[2] = b->E->L.wpb;
+    out[3] = b->E->L.graph;
+    return SHLL_OK;
 }
 
 extern "C" shll_status shll_batch_reset(shll_batch* b) {

@@ -162,9 +162,9 @@ __device__ __forceinline__ void block_reduce(double (&s)[NS], double (&mx)[NM], 
     }
 }
 
-// partial[(inst * gridDim.x + block) * 5 + q]: sums of the DIM+2 conserved variables
+// partial[(inst * gridDim.x + block) * 6 + q]: sums of the DIM+2 conserved variables
 // (unused slots 0) and, at q = 4, the count of cells whose density or energy is
-// non-finite or not positive.
+// non-finite or not positive (q = 5 is an unused maximum slot).
 template <int PREC, int DIM>
 __global__ __launch_bounds__(256) void k_sums(const Inst* insts, const void* s0, const void* s1, int64_t plane,
                                               double* partial) {
@@ -189,22 +189,21 @@ __global__ __launch_bounds__(256) void k_sums(const Inst* insts, const void* s0,
 // Differences of two primitive sets (nv planes, ncell cells from offset off, plane stride):
 // sums |drho|, |du|, |dp| (p = rho*T, R = 1), maxima |drho|/rho_b, |dp|/p_b, |du|, |u_b|,
 // |drho|, |dp|, and a count of non-finite entries.
-__global__ __launch_bounds__(256) void k_compare(const double* __restrict__ a, const double* __restrict__ b,
-                                                 int64_t plane, int64_t off, int64_t ncell, int dim,
-                                                 double* partial) {
+__global__ __launch_bounds__(256) void k_compare(const double* __restrict__ a, int64_t plane_a,
+                                                 const double* __restrict__ b, int64_t plane_b, int64_t ncell,
+                                                 int dim, double* partial) {
     double s[4] = {0, 0, 0, 0};
     double mx[6] = {0, 0, 0, 0, 0, 0};
     const int tv = dim == 2 ? 3 : 2;
     for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < ncell;
          c += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t o = off + c;
-        const double ra = a[o], rb = b[o], ua = a[plane + o], ub = b[plane + o];
-        const double pa = ra * a[tv * plane + o], pb = rb * b[tv * plane + o];
+        const double ra = a[c], rb = b[c], ua = a[plane_a + c], ub = b[plane_b + c];
+        const double pa = ra * a[tv * plane_a + c], pb = rb * b[tv * plane_b + c];
         const double dr = fabs(ra - rb), du = fabs(ua - ub), dp = fabs(pa - pb);
         double dv = 0, vb = 0;
         if (dim == 2) {
-            dv = fabs(a[2 * plane + o] - b[2 * plane + o]);
-            vb = fabs(b[2 * plane + o]);
+            dv = fabs(a[2 * plane_a + c] - b[2 * plane_b + c]);
+            vb = fabs(b[2 * plane_b + c]);
         }
         const bool fin = isfinite(ra) && isfinite(ua) && isfinite(pa) && isfinite(dv);
         s[0] += dr;
@@ -270,9 +269,9 @@ hipError_t launch_sums(int prec, int dim, int nblocks, int n_inst, hipStream_t s
     return sums_p<PREC_C32>(dim, g, st, insts, s0, s1, plane, partial);
 }
 
-hipError_t launch_compare(int nblocks, hipStream_t st, const double* a, const double* b, int64_t plane, int64_t off,
-                          int64_t ncell, int dim, double* partial) {
-    hipLaunchKernelGGL(k_compare, dim3(nblocks), dim3(256), 0, st, a, b, plane, off, ncell, dim, partial);
+hipError_t launch_compare(int nblocks, hipStream_t st, const double* a, int64_t plane_a, const double* b,
+                          int64_t plane_b, int64_t ncell, int dim, double* partial) {
+    hipLaunchKernelGGL(k_compare, dim3(nblocks), dim3(256), 0, st, a, plane_a, b, plane_b, ncell, dim, partial);
     return hipGetLastError();
 }
 

@@ -68,7 +68,8 @@ EXPORTS = [
     "shll_set_launch", "shll_get_launch", "shll_autotune", "shll_set_timing", "shll_last_timing",
     "shll_batch_init", "shll_batch_advance", "shll_batch_run", "shll_batch_get_primitives",
     "shll_batch_get_conserved", "shll_batch_conserved_sums", "shll_batch_set_launch",
-    "shll_batch_autotune", "shll_batch_set_timing", "shll_batch_last_timing",
+    "shll_batch_autotune", "shll_batch_get_launch", "shll_batch_check_state", "shll_batch_compare",
+    "shll_batch_set_timing", "shll_batch_last_timing",
     "shll_batch_total_cell_updates", "shll_batch_reset", "shll_batch_free", "shll_ic_tube",
     "shll_ic_implosion", "shll_ic_four_shocks", "shll_ic_circle", "shll_gen_tube", "shll_gen_domain",
     "shll_ic_domain", "shll_last_error", "shll_version",
@@ -116,6 +117,9 @@ def lib():
         "shll_batch_conserved_sums": (c, [vp, dp]),
         "shll_batch_set_launch": (c, [vp, i32, i32, i32, i32]),
         "shll_batch_autotune": (c, [vp, i32]),
+        "shll_batch_get_launch": (c, [vp, ctypes.POINTER(i32)]),
+        "shll_batch_check_state": (c, [vp, ctypes.POINTER(i64), ctypes.POINTER(i32)]),
+        "shll_batch_compare": (c, [vp, i32, vp, i32, ctypes.POINTER(Errors)]),
         "shll_batch_set_timing": (c, [vp, i32]),
         "shll_batch_last_timing": (c, [vp, dp, ctypes.POINTER(i64), dp]),
         "shll_batch_total_cell_updates": (i64, [vp]),
@@ -330,6 +334,24 @@ class Batch:
 
     def autotune(self, trial_steps):
         _check(lib().shll_batch_autotune(self.h, int(trial_steps)))
+
+    def get_launch(self):
+        out = (ctypes.c_int32 * 4)()
+        _check(lib().shll_batch_get_launch(self.h, out))
+        return tuple(out)
+
+    def check_state(self):
+        """Per-instance count of non-finite / non-positive cells."""
+        bad = np.zeros(len(self.pbs), dtype=np.int64)
+        nb = ctypes.c_int32(0)
+        _check(lib().shll_batch_check_state(self.h, bad.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                            ctypes.byref(nb)))
+        return bad
+
+    def compare(self, k, ref, kref=None):
+        e = Errors()
+        _check(lib().shll_batch_compare(self.h, k, ref.h, k if kref is None else kref, ctypes.byref(e)))
+        return e.as_dict()
 
     def set_timing(self, on=True):
         _check(lib().shll_batch_set_timing(self.h, 1 if on else 0))
