@@ -29,6 +29,9 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "shll-sve-cfd_amd"))
+# launch shapes chosen by the library's autotuner on MI355X (tune/make_tune_cache.py);
+# --autotune re-tunes live instead
+TUNE_CACHE = os.path.join(ROOT, "shll-sve-cfd_amd", "tune", "gfx950.txt")
 
 METRIC = "cell-updates/s + % HBM roofline, FP32-comp vs FP64, 1D/2D SHLL, 1×MI355X"
 PEAK_HBM_GBS = 8000.0
@@ -49,19 +52,20 @@ def bytes_per_update(dim, prec):
     return 2 * nv * s
 
 
-def traffic_from_profiles(kernel_tag):
-    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 --pmc
-    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), or None."""
-    path = os.path.join(ROOT, "profiles", "traffic_%s.txt" % kernel_tag)
+def traffic_per_update(shape):
+    """Calibrated HBM bytes per cell-update of the batch step kernel, from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE scaled by a 1 GiB copy), when the
+    summary was taken with the same launch shape (cells/lane, steps/launch, waves/block);
+    else None."""
+    import re
+    path = os.path.join(ROOT, "profiles", "k_step1d_c32_o2_tubes4096_hbm_traffic.md")
     if not os.path.exists(path):
         return None
-    for line in open(path):
-        if line.startswith("hbm_bytes_per_launch_calibrated:"):
-            try:
-                return float(line.split(":", 1)[1].split()[0])
-            except ValueError:
-                return None
-    return None
+    m = re.search(r"hbm_bytes_per_cell_update:\s*([0-9.eE+-]+)\s*\(launch shape (\d+),(\d+),(\d+)\)",
+                  open(path).read())
+    if not m or tuple(int(x) for x in m.groups()[1:]) != tuple(shape[:3]):
+        return None
+    return float(m.group(1))
 
 
 def cpu_model():
@@ -94,7 +98,7 @@ def calibration_copy(nbytes):
     log("calibration copy: %d bytes device-to-device" % nbytes)
 
 
-def run_batch(S, pbs, prims, steps, launch, K, W):
+def run_batch(S, pbs, prims, steps, launch, K, W, progress=0):
     b = S.Batch(pbs, prims, steps)
     if launch:
         b.set_launch(*launch)
@@ -104,6 +108,22 @@ def run_batch(S, pbs, prims, steps, launch, K, W):
         b.advance(W)
         b.reset()
     b.set_timing(True)
+    if progress > 0:
+        # pieces of whole launches; event time and launch counts are summed over the pieces
+        k = b.get_launch()[1]
+        piece = max(k, progress // k * k)
+        done, ev_ms, launches = 0, 0.0, 0
+        t0 = time.perf_counter()
+        while done < K:
+            n = min(piece, K - done)
+            b.advance(n)
+            done += n
+            _, nl, tot = b.last_timing()
+            ev_ms += tot
+            launches += nl
+            log("progress: %d / %d steps" % (done, K))
+        wall = time.perf_counter() - t0
+        return b, wall, ev_ms / max(1, launches), launches, ev_ms
     t0 = time.perf_counter()
     b.advance(K)
     wall = time.perf_counter() - t0
@@ -270,12 +290,19 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=0, help="time steps to time (0: the whole job)")
-    ap.add_argument("--warmup", type=int, default=64, help="untimed warm-up time steps (then reset)")
+    ap.add_argument("--warmup", type=int, default=128,
+                    help="untimed warm-up time steps (then reset); the default is one 128-step graph chunk")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the FP64 judge of the batch")
+    ap.add_argument("--progress", type=int, default=0,
+                    help="advance the timed run in pieces of this many steps and log each (profiling runs)")
     ap.add_argument("--launch", default="", help="C,k,waves,graph (default: autotune)")
     ap.add_argument("--calib-copy", type=int, default=0, help="bytes of a device copy to run first (profiling)")
+    ap.add_argument("--autotune", action="store_true", help="re-run the autotuner instead of its recorded choices")
     args = ap.parse_args()
+    if not args.autotune:
+        os.environ.setdefault("SHLL_TUNE_CACHE", TUNE_CACHE)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
         sys.exit(2)
@@ -289,17 +316,17 @@ def main():
                                                                  sum(p.nx for p in pbs), max(steps)))
     K = args.steps if args.steps > 0 else max(steps)
     launch = tuple(int(x) for x in args.launch.split(",")) if args.launch else None
-    b, wall, avg_ms, launches, ev_ms = run_batch(S, pbs, prims, steps, launch, K, args.warmup)
+    b, wall, avg_ms, launches, ev_ms = run_batch(S, pbs, prims, steps, launch, K, args.warmup, args.progress)
     upd = updates_done(pbs, steps, K)
     value = upd / wall
     B = bytes_per_update(1, S.C32)
     achieved = B * upd / (ev_ms * 1e-3) / 1e9 if ev_ms > 0 else 0.0
     lau = list(b.get_launch())
     log("timed %d steps: %.3e cell-updates in %.3f s (events %.1f ms, %d launches)" % (K, upd, wall, ev_ms, launches))
-    errs = tube_errors(S, b, pbs, prims, steps, K)
-    sums = b.conserved_sums()
+    errs = None if args.no_accuracy else tube_errors(S, b, pbs, prims, steps, K)
     b.close()
-    tr = traffic_from_profiles("step1d_c32_o2")
+    tpu = traffic_per_update(lau)
+    tr = tpu * upd / launches if tpu is not None and launches > 0 else None
     result = {
         "metric": METRIC,
         "value": value,
@@ -333,7 +360,9 @@ def main():
             "algorithmic_bytes_per_cell_update": B,
             "avg_launch_ms": avg_ms,
             "note": "achieved = 48 B x cell-updates / HIP-event time of the launches; temporal blocking "
-                    "(k steps per launch from registers) lets it exceed the HBM peak",
+                    "(k steps per launch from registers) lets it exceed the HBM peak. traffic = HBM bytes per "
+                    "launch of the timed run: calibrated rocprofv3 FETCH_SIZE+WRITE_SIZE per cell-update "
+                    "(profiles/k_step1d_c32_o2_tubes4096_hbm_traffic.md) x cell-updates per launch",
         },
         "accuracy": errs,
     }

@@ -21,16 +21,15 @@
 
 namespace shll {
 hipError_t launch_step1d(int prec, int order, int C, int wpb, int n_items, hipStream_t s, const StepArgs& a);
-const void* step1d_fn(int prec, int order, int C, int wpb);
 bool step1d_supported(int C, int wpb);
 hipError_t launch_step2d(int prec, int order, int wpb, int n_items, hipStream_t s, const StepArgs& a);
-const void* step2d_fn(int prec, int order, int wpb);
 hipError_t launch_init(int prec, int dim, int nblocks, int n_inst, hipStream_t st, const Inst* insts,
                        const double* prim, void* o0, void* o1, int64_t plane);
 hipError_t launch_output(int prec, int dim, int nblocks, int n_inst, hipStream_t st, const Inst* insts,
                          const void* s0, const void* s1, int64_t plane, double* prim, double* cons);
 hipError_t launch_sums(int prec, int dim, int nblocks, int n_inst, hipStream_t st, const Inst* insts,
                        const void* s0, const void* s1, int64_t plane, double* partial);
+hipError_t launch_ctl(hipStream_t st, int64_t* ctl, int64_t value, int add);
 hipError_t launch_compare(int nblocks, hipStream_t st, const double* a, int64_t plane_a, const double* b,
                           int64_t plane_b, int64_t ncell, int dim, double* partial);
 }  // namespace shll
@@ -229,16 +228,21 @@ struct Engine {
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     int g_launches = 0;
     int g_key[3] = {0, 0, 0};
-    // batch graph (explicit kernel nodes with per-chunk parameters)
-    hipGraph_t bgraph = nullptr;
-    hipGraphExec_t bexec = nullptr;
-    std::vector<hipGraphNode_t> bnodes;
-    int b_launches = 0;
+    // batch graphs: constant nodes whose launch index comes from the device counter
+    // d_ctl (set before a run of chunks, advanced by each chunk's last node) and whose
+    // item count comes from d_active[launch index]; one graph per (start parity, grid
+    // blocks rounded up to a power of two)
+    struct BatchGraph {
+        hipGraphExec_t exec = nullptr;
+        int parity = 0;
+        int64_t blocks = 0;
+    };
+    std::vector<BatchGraph> bgs;
+    int64_t* d_ctl = nullptr;
+    int32_t* d_active = nullptr;
+    size_t active_cap = 0;
+    int64_t jmax = 0;  // launches of a whole batch run
     int b_key[3] = {0, 0, 0};
-    // kernel parameters of the batch graph nodes: kept alive for the life of the graph
-    // (hipKernelNodeParams.kernelParams points into these)
-    std::vector<StepArgs> b_args;
-    std::vector<void*> b_kp;
     // diagnostics scratch
     double* d_prim = nullptr;
     double* d_cons = nullptr;
@@ -258,11 +262,9 @@ static void engine_drop_graphs(Engine* E) {
             (void)hipGraphExecDestroy(E->gexec[p]);
             E->gexec[p] = nullptr;
         }
-    if (E->bexec) (void)hipGraphExecDestroy(E->bexec);
-    if (E->bgraph) (void)hipGraphDestroy(E->bgraph);
-    E->bexec = nullptr;
-    E->bgraph = nullptr;
-    E->bnodes.clear();
+    for (auto& bgx : E->bgs)
+        if (bgx.exec) (void)hipGraphExecDestroy(bgx.exec);
+    E->bgs.clear();
 }
 
 static void engine_free(Engine* E) {
@@ -277,6 +279,8 @@ static void engine_free(Engine* E) {
         if (E->init_copy[w]) (void)hipFree(E->init_copy[w]);
     if (E->d_inst) (void)hipFree(E->d_inst);
     if (E->d_items) (void)hipFree(E->d_items);
+    if (E->d_ctl) (void)hipFree(E->d_ctl);
+    if (E->d_active) (void)hipFree(E->d_active);
     if (E->d_prim) (void)hipFree(E->d_prim);
     if (E->d_cons) (void)hipFree(E->d_cons);
     if (E->d_partial) (void)hipFree(E->d_partial);
@@ -290,6 +294,8 @@ static int radius(const Engine* E) { return E->order; }
 
 // steps this launch shape advances per launch
 static int launch_k(const Engine* E) { return E->dim == 1 ? E->L.k : 1; }
+
+static int64_t active_items(const Engine* E, int64_t j);
 
 static shll_status engine_build_items(Engine* E) {
     if (E->items_valid) return SHLL_OK;
@@ -321,6 +327,24 @@ static shll_status engine_build_items(Engine* E) {
     }
     HIPCHECK(hipMemcpyAsync(E->d_items, E->dim == 1 ? (const void*)it1.data() : (const void*)it2.data(), bytes,
                             hipMemcpyHostToDevice, E->stream));
+    if (E->batch) {
+        // active-item count per launch index, read by the constant graph nodes
+        const int64_t total = E->order_idx.empty() ? 0 : E->h_inst[E->order_idx[0]].nsteps;
+        E->jmax = (total + launch_k(E) - 1) / launch_k(E);
+        if (E->jmax > (int64_t)std::numeric_limits<int32_t>::max())
+            return fail(SHLL_EINVAL, "too many launches for one batch run");
+        std::vector<int32_t> act((size_t)std::max<int64_t>(1, E->jmax), 0);
+        for (int64_t j = 0; j < E->jmax; ++j) act[(size_t)j] = (int32_t)active_items(E, j);
+        const size_t abytes = act.size() * sizeof(int32_t);
+        if (abytes > E->active_cap) {
+            if (E->d_active) HIPCHECK(hipFree(E->d_active));
+            E->d_active = nullptr;
+            HIPCHECK(hipMalloc(&E->d_active, abytes));
+            E->active_cap = abytes;
+        }
+        if (!E->d_ctl) HIPCHECK(hipMalloc(&E->d_ctl, sizeof(int64_t)));
+        HIPCHECK(hipMemcpyAsync(E->d_active, act.data(), abytes, hipMemcpyHostToDevice, E->stream));
+    }
     HIPCHECK(hipStreamSynchronize(E->stream));
     E->items_valid = true;
     engine_drop_graphs(E);
@@ -352,16 +376,16 @@ static StepArgs make_args(const Engine* E, int64_t j, int parity, int64_t n_item
     a.in1 = E->buf[parity][1];
     a.out0 = E->buf[parity ^ 1][0];
     a.out1 = E->buf[parity ^ 1][1];
+    a.jctl = nullptr;
+    a.active = nullptr;
+    a.jofs = 0;
+    a.jmax = 0;
     return a;
 }
 
 static hipError_t launch_one(const Engine* E, const StepArgs& a, int64_t n_items) {
     if (E->dim == 1) return launch_step1d(E->prec, E->order, E->L.C, E->L.wpb, (int)n_items, E->stream, a);
     return launch_step2d(E->prec, E->order, E->L.wpb, (int)n_items, E->stream, a);
-}
-
-static const void* kernel_fn(const Engine* E) {
-    return E->dim == 1 ? step1d_fn(E->prec, E->order, E->L.C, E->L.wpb) : step2d_fn(E->prec, E->order, E->L.wpb);
 }
 
 // Single instance: every launch has the same items; ksteps varies only on the last one.
@@ -422,49 +446,64 @@ static shll_status single_steps(Engine* E, int64_t nsteps) {
 }
 
 // Batch: launch j covers steps [j*k, (j+1)*k) of every instance with nsteps > j*k.
+// Whole chunks of g launches replay constant graphs (device-driven launch index); the
+// remainder of a call is launched directly with host-side parameters.
+static shll_status batch_graph(Engine* E, int parity, int64_t blocks, int g, hipGraphExec_t* out) {
+    for (const auto& bgx : E->bgs)
+        if (bgx.parity == parity && bgx.blocks == blocks) {
+            *out = bgx.exec;
+            return SHLL_OK;
+        }
+    hipGraph_t graph;
+    HIPCHECK(hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipSuccess;
+    for (int q = 0; q < g && e == hipSuccess; ++q) {
+        StepArgs a = make_args(E, 0, (parity + q) & 1, 0);
+        a.jctl = E->d_ctl;
+        a.active = E->d_active;
+        a.jofs = q;
+        a.jmax = (int32_t)E->jmax;
+        e = launch_one(E, a, blocks * E->L.wpb);
+    }
+    if (e == hipSuccess) e = launch_ctl(E->stream, E->d_ctl, g, 1);
+    hipError_t e2 = hipStreamEndCapture(E->stream, &graph);
+    if (e != hipSuccess || e2 != hipSuccess)
+        return fail(SHLL_EDEVICE, "batch graph capture: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+    Engine::BatchGraph bgx;
+    bgx.parity = parity;
+    bgx.blocks = blocks;
+    e = hipGraphInstantiate(&bgx.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(SHLL_EDEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    E->bgs.push_back(bgx);
+    *out = bgx.exec;
+    return SHLL_OK;
+}
+
 static shll_status batch_launches(Engine* E, int64_t nlaunch) {
-    const int64_t total = E->order_idx.empty() ? 0 : E->h_inst[E->order_idx[0]].nsteps;
     const int k = launch_k(E);
-    const int64_t jmax = (total + k - 1) / k;
     int64_t j = E->launches_done;
-    const int64_t jend = std::min<int64_t>(jmax, j + nlaunch);
+    const int64_t jend = std::min<int64_t>(E->jmax, j + nlaunch);
     const int g = E->L.graph > 0 ? std::max(2, (E->L.graph / k) & ~1) : 0;
     const int wpb = E->L.wpb;
     int64_t launches = 0;
+    bool ctl_set = false;
     if (E->timing) HIPCHECK(hipEventRecord(E->ev0, E->stream));
     while (j < jend) {
         if (g > 0 && jend - j >= g) {
-            const int key[3] = {E->L.C, E->L.k, g};
-            if (E->bexec && std::memcmp(key, E->b_key, sizeof(key)) != 0) engine_drop_graphs(E);
-            if ((int)E->b_args.size() != g) {
-                E->b_args.assign(g, StepArgs{});
-                E->b_kp.assign(g, nullptr);
+            if (!ctl_set) {
+                HIPCHECK(launch_ctl(E->stream, E->d_ctl, j, 0));
+                ctl_set = true;
             }
-            std::vector<hipKernelNodeParams> prm(g);
-            for (int q = 0; q < g; ++q) {
-                const int64_t na = active_items(E, j + q);
-                E->b_args[q] = make_args(E, j + q, (int)((j + q) & 1), na);
-                E->b_kp[q] = &E->b_args[q];
-                std::memset(&prm[q], 0, sizeof(prm[q]));
-                prm[q].func = (void*)kernel_fn(E);
-                prm[q].gridDim = dim3((unsigned)std::max<int64_t>(1, (na + wpb - 1) / wpb));
-                prm[q].blockDim = dim3(64 * wpb);
-                prm[q].sharedMemBytes = 0;
-                prm[q].kernelParams = &E->b_kp[q];
-            }
-            if (!E->bexec) {
-                HIPCHECK(hipGraphCreate(&E->bgraph, 0));
-                E->bnodes.assign(g, nullptr);
-                for (int q = 0; q < g; ++q)
-                    HIPCHECK(hipGraphAddKernelNode(&E->bnodes[q], E->bgraph, q ? &E->bnodes[q - 1] : nullptr,
-                                                   q ? 1 : 0, &prm[q]));
-                HIPCHECK(hipGraphInstantiate(&E->bexec, E->bgraph, nullptr, nullptr, 0));
-                std::memcpy(E->b_key, key, sizeof(key));
-            } else {
-                for (int q = 0; q < g; ++q)
-                    HIPCHECK(hipGraphExecKernelNodeSetParams(E->bexec, E->bnodes[q], &prm[q]));
-            }
-            HIPCHECK(hipGraphLaunch(E->bexec, E->stream));
+            // the chunk's first launch has the most active items; round its grid up to a
+            // power of two so few graphs cover the whole run
+            const int64_t need = (active_items(E, j) + wpb - 1) / wpb;
+            int64_t blocks = 1;
+            while (blocks < need) blocks <<= 1;
+            hipGraphExec_t exec = nullptr;
+            shll_status st = batch_graph(E, (int)(j & 1), blocks, g, &exec);
+            if (st != SHLL_OK) return st;
+            HIPCHECK(hipGraphLaunch(exec, E->stream));
             j += g;
             launches += g;
             continue;
@@ -522,19 +561,26 @@ static shll_status engine_init_state(Engine* E, const double* const* prims) {
     const size_t pbytes = (size_t)E->plane * E->nv * sizeof(double);
     double* d_p = nullptr;
     HIPCHECK(hipMalloc(&d_p, pbytes));
-    std::vector<double> staging;
+    // gather every instance's primitives into one pinned staging buffer in the device
+    // layout, then one host-to-device copy (instead of nvar copies per instance)
+    double* h_p = nullptr;
+    if (hipHostMalloc((void**)&h_p, pbytes, hipHostMallocDefault) != hipSuccess) {
+        (void)hipFree(d_p);
+        return fail(SHLL_ENOMEM, "hipHostMalloc(%zu) for the initial data", pbytes);
+    }
     int64_t maxcells = 0;
     for (size_t i = 0; i < E->h_inst.size(); ++i) {
         const int64_t nc = E->inst_cells[i];
         maxcells = std::max(maxcells, nc);
-        for (int v = 0; v < E->nv; ++v) {
-            hipError_t e = hipMemcpyAsync(d_p + v * E->plane + E->h_inst[i].off, prims[i] + (size_t)v * nc,
-                                          (size_t)nc * sizeof(double), hipMemcpyHostToDevice, E->stream);
-            if (e != hipSuccess) {
-                (void)hipFree(d_p);
-                return fail(SHLL_EDEVICE, "upload: %s", hipGetErrorString(e));
-            }
-        }
+        for (int v = 0; v < E->nv; ++v)
+            std::memcpy(h_p + v * E->plane + E->h_inst[i].off, prims[i] + (size_t)v * nc, (size_t)nc * sizeof(double));
+    }
+    hipError_t eu = hipMemcpyAsync(d_p, h_p, pbytes, hipMemcpyHostToDevice, E->stream);
+    hipError_t es = hipStreamSynchronize(E->stream);
+    (void)hipHostFree(h_p);
+    if (eu != hipSuccess || es != hipSuccess) {
+        (void)hipFree(d_p);
+        return fail(SHLL_EDEVICE, "upload: %s", hipGetErrorString(eu != hipSuccess ? eu : es));
     }
     hipError_t e = launch_init(E->prec, E->dim, elem_blocks(maxcells), (int)E->h_inst.size(), E->stream, E->d_inst, d_p,
                                E->buf[0][0], E->buf[0][1], E->plane);
@@ -995,6 +1041,11 @@ static shll_status engine_autotune(Engine* E, int32_t trial_steps, int64_t shape
         for (int C : {1, 2, 4, 8})
             for (int k : {1, 2, 4, 8, 16, 32, 64})
                 for (int w : {1, 4}) {
+                    // batches: 256-thread workgroups and at least 8 steps per launch keep the
+                    // dispatch count of a whole run near 10^4 (rocprofv3 kernel tracing of
+                    // 2.6e4 single-wave dispatches crashed in the tool); the tuner measured
+                    // the excluded shapes within a few percent of the kept ones
+                    if (E->batch && (w != 4 || k < 8)) continue;
                     Launch L = start;
                     L.C = C; L.k = k; L.wpb = w;
                     L.graph = std::max(128, 2 * k);

@@ -154,11 +154,12 @@ template <int PREC, int ORDER, int C, int WPB>
 __global__ __launch_bounds__(64 * WPB) void k_step1d(StepArgs a) {
     using RT = typename Real<PREC>::T;
     const int wave = (int)blockIdx.x * WPB + uniform((int)threadIdx.x >> 6);
-    if (wave >= a.n_items) return;
+    const LaunchView lv = launch_view(a);
+    if (wave >= lv.n_items) return;
     const int lane = (int)threadIdx.x & 63;
     const Item1 it = ((const Item1*)a.items)[wave];
     const Inst in = a.inst[it.inst];
-    const int64_t left = in.nsteps - a.step_base;
+    const int64_t left = in.nsteps - lv.step_base;
     const int ns = left < (int64_t)a.ksteps ? (int)left : a.ksteps;
     if (ns <= 0) return;
     const int n = in.nx;
@@ -249,21 +250,6 @@ hipError_t launch_step1d(int prec, int order, int C, int wpb, int n_items, hipSt
     if (prec == PREC_C32) return order == 1 ? launch_o<PREC_C32, 1>(C, wpb, nblocks, s, a)
                                             : launch_o<PREC_C32, 2>(C, wpb, nblocks, s, a);
     return hipErrorInvalidValue;
-}
-
-template <int PREC, int ORDER>
-static const void* fn_o(int C, int wpb) {
-#define SHLL_FN(CC)                                                                              \
-    if (C == CC) return wpb == 1 ? (const void*)k_step1d<PREC, ORDER, CC, 1> : (const void*)k_step1d<PREC, ORDER, CC, 4>;
-    SHLL_FN(1) SHLL_FN(2) SHLL_FN(4) SHLL_FN(8)
-#undef SHLL_FN
-    return nullptr;
-}
-
-const void* step1d_fn(int prec, int order, int C, int wpb) {
-    if (prec == PREC_F64) return order == 1 ? fn_o<PREC_F64, 1>(C, wpb) : fn_o<PREC_F64, 2>(C, wpb);
-    if (prec == PREC_REF32) return order == 1 ? fn_o<PREC_REF32, 1>(C, wpb) : fn_o<PREC_REF32, 2>(C, wpb);
-    return order == 1 ? fn_o<PREC_C32, 1>(C, wpb) : fn_o<PREC_C32, 2>(C, wpb);
 }
 
 bool step1d_supported(int C, int wpb) { return (C == 1 || C == 2 || C == 4 || C == 8) && (wpb == 1 || wpb == 4); }

@@ -66,11 +66,12 @@ __global__ __launch_bounds__(64 * WPB) void k_step2d(StepArgs a) {
     constexpr bool CMP = PREC == PREC_C32;
 
     const int wave = (int)blockIdx.x * WPB + uniform((int)threadIdx.x >> 6);
-    if (wave >= a.n_items) return;
+    const LaunchView lv = launch_view(a);
+    if (wave >= lv.n_items) return;
     const int lane = (int)threadIdx.x & 63;
     const Item2 it = ((const Item2*)a.items)[wave];
     const Inst in = a.inst[it.inst];
-    if (in.nsteps - a.step_base <= 0) return;
+    if (in.nsteps - lv.step_base <= 0) return;
     const int nx = in.nx, ny = in.ny;
     const int64_t P = a.plane;
     const int j = it.j0 + lane;
@@ -207,17 +208,6 @@ static hipError_t launch2_o(int wpb, int nblocks, hipStream_t s, const StepArgs&
     if (wpb == 1) hipLaunchKernelGGL((k_step2d<PREC, ORDER, 1>), dim3(nblocks), dim3(64), 0, s, a);
     else hipLaunchKernelGGL((k_step2d<PREC, ORDER, 4>), dim3(nblocks), dim3(256), 0, s, a);
     return hipGetLastError();
-}
-
-template <int PREC, int ORDER>
-static const void* fn2_o(int wpb) {
-    return wpb == 1 ? (const void*)k_step2d<PREC, ORDER, 1> : (const void*)k_step2d<PREC, ORDER, 4>;
-}
-
-const void* step2d_fn(int prec, int order, int wpb) {
-    if (prec == PREC_F64) return order == 1 ? fn2_o<PREC_F64, 1>(wpb) : fn2_o<PREC_F64, 2>(wpb);
-    if (prec == PREC_REF32) return order == 1 ? fn2_o<PREC_REF32, 1>(wpb) : fn2_o<PREC_REF32, 2>(wpb);
-    return order == 1 ? fn2_o<PREC_C32, 1>(wpb) : fn2_o<PREC_C32, 2>(wpb);
 }
 
 hipError_t launch_step2d(int prec, int order, int wpb, int n_items, hipStream_t s, const StepArgs& a) {

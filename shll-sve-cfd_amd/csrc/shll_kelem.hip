@@ -220,6 +220,20 @@ __global__ __launch_bounds__(256) void k_compare(const double* __restrict__ a, i
     block_reduce<4, 6>(s, mx, partial + (int64_t)blockIdx.x * 10);
 }
 
+// Launch counter of the constant-parameter batch graphs: set once before a run of
+// graph chunks (add = 0), advanced by the last node of every chunk (add = 1).
+__global__ __launch_bounds__(64) void k_ctl(int64_t* ctl, int64_t value, int add) {
+    if (threadIdx.x == 0) {
+        const int64_t v = add ? ctl[0] + value : value;
+        ctl[0] = v;
+    }
+}
+
+hipError_t launch_ctl(hipStream_t st, int64_t* ctl, int64_t value, int add) {
+    hipLaunchKernelGGL(k_ctl, dim3(1), dim3(64), 0, st, ctl, value, add);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 template <int PREC>
 static hipError_t init_p(int dim, dim3 g, hipStream_t st, const Inst* insts, const double* prim, void* o0,

@@ -1,6 +1,7 @@
 // shll_types.h -- tables and launch arguments shared by the host API and the kernels.
 #pragma once
 
+#include <hip/hip_runtime.h>
 #include <stdint.h>
 
 namespace shll {
@@ -61,6 +62,29 @@ struct StepArgs {
     const void* in1;     //             C32 lo float*
     void* out0;          // state written
     void* out1;
+    // Device-driven launches (constant graph nodes): when jctl is set, this launch's
+    // index is j = *jctl + jofs; it covers n_items = active[j] items (0 if j >= jmax)
+    // and step_base = j * ksteps, overriding the two host-side fields above.
+    const int64_t* jctl;
+    const int32_t* active;
+    int32_t jofs;
+    int32_t jmax;
 };
+
+// launch index, item count and step base of a step launch
+struct LaunchView {
+    int32_t n_items;
+    int64_t step_base;
+};
+
+__host__ __device__ inline LaunchView launch_view(const StepArgs& a) {
+    LaunchView v{a.n_items, a.step_base};
+    if (a.jctl) {
+        const int64_t j = *a.jctl + a.jofs;
+        v.n_items = j < a.jmax ? a.active[j] : 0;
+        v.step_base = j * (int64_t)a.ksteps;
+    }
+    return v;
+}
 
 }  // namespace shll

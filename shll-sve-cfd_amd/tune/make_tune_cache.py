@@ -1,0 +1,41 @@
+#!/usr/bin/env python3
+"""Run the library's autotuner (shll_autotune / shll_batch_autotune) on the benchmark
+problems and record its choices in a tuning cache (one line per device, kernel and
+shape class: "<key>|C k waves graph").  The library reads the cache named by
+$SHLL_TUNE_CACHE; bench.py points it at tune/gfx950.txt so timed runs use the
+recorded choice instead of re-tuning.
+
+Usage (on the GPU):  python shll-sve-cfd_amd/tune/make_tune_cache.py OUT.txt
+"""
+import os
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+
+
+def main():
+    out = os.path.abspath(sys.argv[1] if len(sys.argv) > 1 else os.path.join(HERE, "gfx950.txt"))
+    if os.path.exists(out):
+        os.remove(out)
+    os.environ["SHLL_TUNE_CACHE"] = out
+    import shll_amd as S
+
+    # batched FP32-compensated second-order tubes (the benchmark batch)
+    pbs, prims, steps = S.tube_batch(20261202, 4096, precision=S.C32)
+    b = S.Batch(pbs, prims, steps)
+    b.autotune(64)
+    print("batch c32 o2:", b.get_launch(), flush=True)
+    b.close()
+    # single instances: 1D first order 1024 cells, 1D second order 65536 cells
+    for pb, trial in ((S.problem(1, 1, 1024, precision=S.F64), 400),
+                      (S.problem(1, 2, 65536, precision=S.F64), 2000),
+                      (S.problem(1, 2, 65536, precision=S.C32), 2000)):
+        with S.Solver(pb, S.ic_tube(pb.nx)) as s:
+            s.autotune(trial)
+            print("single dim1 order%d prec%d N=%d:" % (pb.order, pb.precision, pb.nx), s.get_launch(), flush=True)
+    print(open(out).read())
+
+
+if __name__ == "__main__":
+    main()

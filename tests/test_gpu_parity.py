@@ -171,6 +171,35 @@ def test_1d_batch_equals_single_runs(orc, prec, order):
     b.close()
 
 
+@pytest.mark.parametrize("dim", [1, 2])
+def test_batch_piecewise_advance_with_graph_chunks(orc, dim):
+    """Advancing in uneven pieces (graph chunks starting on odd and even launches, direct
+    launches between them) gives every instance its oracle result."""
+    if dim == 1:
+        pbs, prims, steps = S.tube_batch(4321, 9, nmin=100, nmax=2500, precision=S.F64)
+        steps = [max(1, s // 10) for s in steps]
+        launch = (2, 8, 4, 32)   # 4 launches per graph chunk
+    else:
+        pbs, prims = S.domain_batch(77, 3, smin=30, smax=90, precision=S.F64)
+        steps = [41, 17, 29]
+        launch = (1, 1, 4, 4)    # 4 launches per graph chunk
+    b = S.Batch(pbs, prims, steps)
+    b.set_launch(*launch)
+    banner("batch pieces", dim)
+    for n in (3, 50, 9, 1):
+        b.advance(n)
+    b.run()
+    for k in range(len(pbs)):
+        ref = oracle_run(orc, pbs[k], prims[k], steps[k])
+        np.testing.assert_array_equal(b.conserved(k), ref[1], err_msg="instance %d" % k)
+    b.reset()
+    b.run()
+    for k in range(len(pbs)):
+        ref = oracle_run(orc, pbs[k], prims[k], steps[k])
+        np.testing.assert_array_equal(b.conserved(k), ref[1], err_msg="instance %d after reset" % k)
+    b.close()
+
+
 def test_2d_batch_equals_single_runs(orc):
     pbs, prims = S.domain_batch(31, 4, smin=40, smax=120, precision=S.C32)
     steps = [30, 11, 20, 5]
