@@ -66,7 +66,7 @@ def test_ref32_matches_reference(orc, name, case):
             p, u, a = (st["prim"].astype(np.float32), st["cons"].astype(np.float32),
                        st["a"].astype(np.float32))
         else:
-            po, co = orc.run(pb, orc.REF32, ic, k)
+            po, co = orc.run(pb, orc.REF32, ic, k, nthreads=4)
             p, u, a = po.astype(np.float32), co.astype(np.float32), None
         if case["scheme"] == "1d_o1":
             np.testing.assert_array_equal(p, g["p_%d" % k], err_msg="%s p step %d" % (name, k))
