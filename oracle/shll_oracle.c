@@ -109,10 +109,15 @@ static void two_sum(float a, float b, float *s, float *e)
     *s = ss;
 }
 
+/* C32 minmod: the median of (l, r, 0), i.e. the smaller magnitude when l and r
+ * share a sign and 0 otherwise (the GPU evaluates it as one median instruction).
+ * It differs from the reference's product test only where l*r underflows and in
+ * the sign of a zero result, which never changes a face flux sum. */
 static float mm_c32(float l, float r)
 {
-    if (l * r < 0.0f) return 0.0f;
-    return fabsf(l) < fabsf(r) ? l : r;
+    if (l > 0.0f && r > 0.0f) return l < r ? l : r;
+    if (l < 0.0f && r < 0.0f) return l > r ? l : r;
+    return 0.0f;
 }
 
 static int c32_alloc(c32grid *g, const orc_problem *pb)

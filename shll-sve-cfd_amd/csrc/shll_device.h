@@ -112,10 +112,12 @@ struct C32Arith {
         z2 = (0.5f * a) * (1.0f - M * M);
         z3 = 0.5f * (M - 1.0f);
     }
-    __device__ static __forceinline__ float mm(float l, float r) {
-        float pick = fabsf(l) < fabsf(r) ? l : r;
-        return l * r < 0.0f ? 0.0f : pick;
-    }
+    // minmod as the median of (l, r, 0): the smaller magnitude when the signs agree,
+    // zero otherwise -- one v_med3_f32.  Unlike the reference's product test it also
+    // clips slopes of opposite sign whose float product underflows, and the sign of a
+    // zero result is unspecified (a zero slope only ever meets a nonzero or +0 face
+    // flux, where the sign of the zero cannot change the sum).
+    __device__ static __forceinline__ float mm(float l, float r) { return __builtin_amdgcn_fmed3f(l, r, 0.0f); }
 };
 
 // TwoSum (Knuth, TAOCP vol. 2, 4.2.2): s + e == a + b exactly

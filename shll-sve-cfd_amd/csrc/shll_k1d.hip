@@ -24,9 +24,12 @@ template <> struct Real<PREC_C32> { using T = float; };
 
 // slope of a split-flux window at position p (window index p+2) of global cell gg;
 // zero on the two boundary cells (2nd_order_base_shll.c:227-256)
-template <class A, class RT, int NW>
+// WALLS = false: the wavefront's segment is at least 3 cells from both ends, so no
+// boundary rule can apply (wave-uniform choice, see k_step1d)
+template <class A, bool WALLS, class RT, int NW>
 __device__ __forceinline__ RT slope_at(const RT (&W)[NW], int p, int gg, int n) {
     RT s = A::mm(W[p + 2] - W[p + 1], W[p + 3] - W[p + 2]);
+    if constexpr (!WALLS) return s;
     return (gg <= 0 || gg >= n - 1) ? (RT)0 : s;
 }
 
@@ -51,13 +54,12 @@ __device__ __forceinline__ void make_window(const RT (&q)[C], RT (&W)[C + 4]) {
 // ---------------------------------------------------------------------------
 // F64 / REF32: the reference association, cell by cell
 // ---------------------------------------------------------------------------
-template <int PREC, int ORDER, int C>
+template <int PREC, int ORDER, int C, bool WALLS>
 __device__ __forceinline__ void step_ref(typename Real<PREC>::T (&u)[3][C], const Inst& in, int gl) {
     using RT = typename Real<PREC>::T;
     using A = RefArith<RT>;
     const RT cv = PREC == PREC_F64 ? (RT)in.k.d_cv : (RT)in.k.f_cv;
     const RT gR = PREC == PREC_F64 ? (RT)in.k.d_gR : (RT)in.k.f_gR;
-    const RT R = PREC == PREC_F64 ? (RT)in.k.d_R : (RT)in.k.f_R;
     const RT lam = PREC == PREC_F64 ? (RT)in.k.d_lamx : (RT)in.k.f_lamx;
     const int n = in.nx;
     RT fp[3][C], fm[3][C];
@@ -66,7 +68,7 @@ __device__ __forceinline__ void step_ref(typename Real<PREC>::T (&u)[3][C], cons
         RT vx, T, a, z1, z2, z3;
         A::prim1(u[0][m], u[1][m], u[2][m], in.ke, cv, gR, vx, T, a);
         A::zinv(vx / a, a, z1, z2, z3);
-        RT P = u[0][m] * R * T;
+        RT P = u[0][m] * T;  // (rho*R)*T with R = 1 (base_shll.c:17, :141)
         RT f0 = u[1][m], f1 = u[1][m] * vx + P, f2 = vx * (u[2][m] + P);
         split<RT>(f0, u[0][m], z1, z2, z3, fp[0][m], fm[0][m]);
         split<RT>(f1, u[1][m], z1, z2, z3, fp[1][m], fm[1][m]);
@@ -82,14 +84,21 @@ __device__ __forceinline__ void step_ref(typename Real<PREC>::T (&u)[3][C], cons
 #pragma unroll
         for (int m = 0; m < C; ++m) {
             const int g = gl + m;
-            RT Lv = g == 0 ? (refl ? sgn * fm[v][m] : fp[v][m]) : FP[m + 1];
-            RT Rv = g == n - 1 ? (refl ? sgn * fp[v][m] : fm[v][m]) : FM[m + 3];
+            RT Lv = FP[m + 1], Rv = FM[m + 3];
+            if constexpr (WALLS) {
+                Lv = g == 0 ? (refl ? sgn * fm[v][m] : fp[v][m]) : Lv;
+                Rv = g == n - 1 ? (refl ? sgn * fp[v][m] : fm[v][m]) : Rv;
+            }
             RT uu = u[v][m] - lam * (fp[v][m] - fm[v][m] + Rv - Lv);
             if constexpr (ORDER == 2) {
-                RT dp = slope_at<A>(FP, m, g, n);
-                RT dm = slope_at<A>(FM, m, g, n);
-                RT Ldf = g == 0 ? (RT)0 : slope_at<A>(FP, m - 1, g - 1, n);
-                RT Rdf = g == n - 1 ? (RT)0 : slope_at<A>(FM, m + 1, g + 1, n);
+                RT dp = slope_at<A, WALLS>(FP, m, g, n);
+                RT dm = slope_at<A, WALLS>(FM, m, g, n);
+                RT Ldf = slope_at<A, WALLS>(FP, m - 1, g - 1, n);
+                RT Rdf = slope_at<A, WALLS>(FM, m + 1, g + 1, n);
+                if constexpr (WALLS) {
+                    Ldf = g == 0 ? (RT)0 : Ldf;
+                    Rdf = g == n - 1 ? (RT)0 : Rdf;
+                }
                 uu = (RT)((double)uu - 0.5 * (double)lam * (double)(dp + dm - Rdf - Ldf));
             }
             u[v][m] = uu;
@@ -100,10 +109,10 @@ __device__ __forceinline__ void step_ref(typename Real<PREC>::T (&u)[3][C], cons
 // ---------------------------------------------------------------------------
 // C32: float fluxes of the hi word, face terms t = lam*G added with TwoSum
 // ---------------------------------------------------------------------------
-template <int ORDER, int C>
+template <int ORDER, int C, bool WALLS>
 __device__ __forceinline__ void step_c32(float (&hi)[3][C], float (&lo)[3][C], const Inst& in, int gl) {
     using A = C32Arith;
-    const float rcv = in.k.c_rcv, gR = in.k.c_gR, R = in.k.c_R, lam = in.k.c_lamx;
+    const float rcv = in.k.c_rcv, gR = in.k.c_gR, lam = in.k.c_lamx;
     const int n = in.nx;
     float fp[3][C], fm[3][C];
 #pragma unroll
@@ -111,7 +120,7 @@ __device__ __forceinline__ void step_c32(float (&hi)[3][C], float (&lo)[3][C], c
         float vx, T, a, z1, z2, z3;
         A::prim1(hi[0][m], hi[1][m], hi[2][m], rcv, gR, vx, T, a);
         A::zinv(vx / a, a, z1, z2, z3);
-        float P = hi[0][m] * R * T;
+        float P = hi[0][m] * T;  // R = 1
         float f0 = hi[1][m], f1 = hi[1][m] * vx + P, f2 = vx * (hi[2][m] + P);
         split<float>(f0, hi[0][m], z1, z2, z3, fp[0][m], fm[0][m]);
         split<float>(f1, hi[1][m], z1, z2, z3, fp[1][m], fm[1][m]);
@@ -129,10 +138,12 @@ __device__ __forceinline__ void step_c32(float (&hi)[3][C], float (&lo)[3][C], c
             const int gf = gl + f;  // face gf lies between cells gf-1 and gf
             float G = FP[f + 1] + FM[f + 2];
             if constexpr (ORDER == 2)
-                G = G + 0.5f * (slope_at<A>(FP, f - 1, gf - 1, n) - slope_at<A>(FM, f, gf, n));
-            float Gl = refl ? FM[f + 2] + sgn * FM[f + 2] : FP[f + 2] + FM[f + 2];
-            float Gr = refl ? FP[f + 1] + sgn * FP[f + 1] : FP[f + 1] + FM[f + 1];
-            G = gf == 0 ? Gl : (gf == n ? Gr : G);
+                G = G + 0.5f * (slope_at<A, WALLS>(FP, f - 1, gf - 1, n) - slope_at<A, WALLS>(FM, f, gf, n));
+            if constexpr (WALLS) {
+                float Gl = refl ? FM[f + 2] + sgn * FM[f + 2] : FP[f + 2] + FM[f + 2];
+                float Gr = refl ? FP[f + 1] + sgn * FP[f + 1] : FP[f + 1] + FM[f + 1];
+                G = gf == 0 ? Gl : (gf == n ? Gr : G);
+            }
             t[f] = lam * G;
         }
 #pragma unroll
@@ -165,6 +176,8 @@ __global__ __launch_bounds__(64 * WPB) void k_step1d(StepArgs a) {
     const int n = in.nx;
     const int gl = it.g0 + lane * C;
     const int64_t P = a.plane;
+    // wall rules can only apply if the segment reaches within 3 cells of an end
+    const bool walls = it.g0 <= 2 || it.g0 + 64 * C >= n - 3;
 
     if constexpr (PREC == PREC_C32) {
         const float* __restrict__ ih = (const float*)a.in0 + in.off;
@@ -180,7 +193,10 @@ __global__ __launch_bounds__(64 * WPB) void k_step1d(StepArgs a) {
                 lo[v][m] = ok ? il[v * P + g] : 0.0f;
             }
         }
-        for (int s = 0; s < ns; ++s) step_c32<ORDER, C>(hi, lo, in, gl);
+        if (walls)
+            for (int s = 0; s < ns; ++s) step_c32<ORDER, C, true>(hi, lo, in, gl);
+        else
+            for (int s = 0; s < ns; ++s) step_c32<ORDER, C, false>(hi, lo, in, gl);
         float* __restrict__ oh = (float*)a.out0 + in.off;
         float* __restrict__ ol = (float*)a.out1 + in.off;
 #pragma unroll
@@ -204,7 +220,10 @@ __global__ __launch_bounds__(64 * WPB) void k_step1d(StepArgs a) {
 #pragma unroll
             for (int v = 0; v < 3; ++v) u[v][m] = ok ? iu[v * P + g] : (RT)(v == 0 ? 1.0 : (v == 1 ? 0.0 : 2.5));
         }
-        for (int s = 0; s < ns; ++s) step_ref<PREC, ORDER, C>(u, in, gl);
+        if (walls)
+            for (int s = 0; s < ns; ++s) step_ref<PREC, ORDER, C, true>(u, in, gl);
+        else
+            for (int s = 0; s < ns; ++s) step_ref<PREC, ORDER, C, false>(u, in, gl);
         RT* __restrict__ ou = (RT*)a.out0 + in.off;
 #pragma unroll
         for (int m = 0; m < C; ++m) {

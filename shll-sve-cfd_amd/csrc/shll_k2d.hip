@@ -26,17 +26,14 @@ template <int PREC, class RT>
 __device__ __forceinline__ void fluxes2(const RT (&U)[4], const Inst& in, RT (&xp)[4], RT (&xm)[4], RT (&yp)[4],
                                         RT (&ym)[4]) {
     RT vx, vy, T, a, z1, z2, z3;
-    RT R;
     if constexpr (PREC == PREC_C32) {
         C32Arith::prim2(U[0], U[1], U[2], U[3], in.k.c_rcv, in.k.c_gR, vx, vy, T, a);
-        R = in.k.c_R;
     } else {
         const RT cv = PREC == PREC_F64 ? (RT)in.k.d_cv : (RT)in.k.f_cv;
         const RT gR = PREC == PREC_F64 ? (RT)in.k.d_gR : (RT)in.k.f_gR;
         RefArith<RT>::prim2(U[0], U[1], U[2], U[3], cv, gR, vx, vy, T, a);
-        R = PREC == PREC_F64 ? (RT)in.k.d_R : (RT)in.k.f_R;
     }
-    const RT P = U[0] * R * T;
+    const RT P = U[0] * T;  // (rho*R)*T with R = 1 (base_shll_2d.c:24, :246)
     // x direction (base_shll_2d.c:249-272)
     if constexpr (PREC == PREC_C32) C32Arith::zinv(vx / a, a, z1, z2, z3);
     else RefArith<RT>::zinv(vx / a, a, z1, z2, z3);

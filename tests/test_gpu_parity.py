@@ -44,6 +44,8 @@ def oracle_run(orc, pb, prim, nsteps):
 def assert_same(pb, gpu, ora, what):
     gp, gc, graw = gpu
     op, oc, oraw = ora
+    # compare physical states only: unstable data (CFL > 1) ends in NaN, which says nothing
+    assert np.isfinite(oc).all(), what + ": the oracle run left the stable regime"
     if pb.precision == S.C32:
         np.testing.assert_array_equal(graw[0], oraw[0], err_msg=what + " hi")
         np.testing.assert_array_equal(graw[1], oraw[1], err_msg=what + " lo")
@@ -121,7 +123,7 @@ def test_2d_random_and_circle(orc, prec, order):
     pb = S.problem(2, order, 70, 150, precision=prec)
     ic = icgen.random_2d(70, 150, 9, block=1)
     assert_same(pb, gpu_run(pb, ic, 2), oracle_run(orc, pb, ic, 2), "2d rough")
-    ic = icgen.random_2d(70, 150, 10, block=13)
+    ic = icgen.random_2d(70, 150, 13, block=13)
     assert_same(pb, gpu_run(pb, ic, 25), oracle_run(orc, pb, ic, 25), "2d blocky")
     pb = S.problem(2, order, 96, 96, precision=prec)
     ic = S.ic_circle(96, 96)

@@ -203,9 +203,20 @@ def test_sod_converges_to_the_exact_riemann_solution(orc):
 
 
 def test_minmod_brute_force(orc):
-    """minmod(l, r) = 0 if l*r < 0, else the smaller magnitude, ties to r (all modes)."""
+    """minmod(l, r) = 0 if l*r < 0, else the smaller magnitude, ties to r (reference modes);
+    the C32 limiter is the median of (l, r, 0)."""
     vals = [-3.0, -1.5, -1.0, -0.0, 0.0, 0.5, 1.0, 1.5, 3.0, 1e-30, -1e-30, 2e-25, -3e-25]
-    for mode in (orc.F64, orc.REF32, orc.C32):
+    for l in vals:
+        for r in vals:
+            lf, rf = float(np.float32(l)), float(np.float32(r))
+            want = sorted([lf, rf, 0.0])[1]
+            got = orc.minmod(orc.C32, l, r)
+            assert got == want, (l, r, got, want)
+            if lf * rf > 0:
+                assert got == (lf if abs(lf) < abs(rf) else rf)
+    # opposite signs whose float product underflows are clipped too
+    assert orc.minmod(orc.C32, 1e-30, -1e-30) == 0.0
+    for mode in (orc.F64, orc.REF32):
         f = float if mode == orc.F64 else (lambda x: float(np.float32(x)))
         for l in vals:
             for r in vals:
