@@ -87,11 +87,13 @@ int64_t orc_steps_to_time(const orc_problem *pb, float t_end)
 /* ------------------------------------------------------------------------- */
 /* FP32-compensated mode.                                                    */
 /* Constants are the FP64 oracle's, rounded once to float.  Split fluxes are  */
-/* float functions of the hi word.  The update is regrouped into face terms   */
-/* t = lam*G (one float per face, shared by both cells) that are added to the */
-/* (hi, lo) pair with TwoSum (Knuth, TAOCP vol. 2, 4.2.2), so the state keeps */
-/* ~48 significant bits and sum(hi+lo) telescopes: mass and energy are        */
-/* conserved to ~1e-14 relative with any wall treatment that conserves them.  */
+/* float functions of the hi word (fused multiply-adds where a product feeds  */
+/* a sum).  The update is regrouped into face terms t = lam*G (one float per  */
+/* face, shared by both cells) that are added to the (hi, lo) pair with       */
+/* TwoSum (Knuth, TAOCP vol. 2, 4.2.2) and renormalised with Fast2Sum, so the */
+/* state keeps ~48 significant bits and sum(hi+lo) telescopes: mass and       */
+/* energy are conserved to ~1e-14 relative with any wall treatment that       */
+/* conserves them.                                                            */
 /* ------------------------------------------------------------------------- */
 typedef struct {
     int nx, ny, n, nv, dim, order, bc;
@@ -197,7 +199,28 @@ static void c32_init(c32grid *g, const double *prim)
     }
 }
 
-/* Per-cell float flux of the compensated mode. */
+/* C32 split of one flux component: F+ = F*Z1 + U*Z2, F- = -F*Z3 - U*Z2 with U*Z2
+ * rounded once and each sum fused (fmaf is correctly rounded in C99/glibc). */
+static void c32_split(float f, float U, float z1, float z2, float z3, float *fp, float *fm)
+{
+    float uz = U * z2;
+    *fp = fmaf(f, z1, uz);
+    *fm = fmaf(-f, z3, -uz);
+}
+
+/* z1 = (M+1)/2, z2 = (a/2)(1-M^2), z3 = (M-1)/2 (M/2 is exact, so z1 and z3 round
+ * exactly like 0.5*(M+-1)) */
+static void c32_z(float M, float a, float *z1, float *z2, float *z3)
+{
+    float h = 0.5f * M;
+    *z1 = h + 0.5f;
+    *z2 = (0.5f * a) * fmaf(-M, M, 1.0f);
+    *z3 = h - 0.5f;
+}
+
+/* Per-cell float flux of the compensated mode: the reference's formulas
+ * (base_shll.c:135-157, base_shll_2d.c:240-298) in float, with fused multiply-adds
+ * where a product feeds a sum. */
 static void c32_cell_flux(c32grid *g, int c)
 {
     int v;
@@ -205,41 +228,30 @@ static void c32_cell_flux(c32grid *g, int c)
         float r = g->hi[0][c], m = g->hi[1][c], E = g->hi[2][c];
         float inv = 1.0f / r;
         float vx = m * inv, e = E * inv;
-        float T = (e - 0.5f * (vx * vx)) * g->rcv;
+        float T = fmaf(-0.5f * vx, vx, e) * g->rcv;
         float a = sqrtf(g->gR * T);
-        float M = vx / a;
-        float z1 = 0.5f * (M + 1.0f), z2 = (0.5f * a) * (1.0f - M * M), z3 = 0.5f * (M - 1.0f);
+        float z1, z2, z3;
         float P = (r * g->R) * T;
         float f[3], U[3];
-        f[0] = m; f[1] = m * vx + P; f[2] = vx * (E + P);
+        c32_z(vx / a, a, &z1, &z2, &z3);
+        f[0] = m; f[1] = fmaf(m, vx, P); f[2] = vx * (E + P);
         U[0] = r; U[1] = m; U[2] = E;
-        for (v = 0; v < 3; v++) {
-            g->fp[v][c] = f[v] * z1 + U[v] * z2;
-            g->fm[v][c] = -f[v] * z3 - U[v] * z2;
-        }
+        for (v = 0; v < 3; v++) c32_split(f[v], U[v], z1, z2, z3, &g->fp[v][c], &g->fm[v][c]);
     } else {
         float r = g->hi[0][c], mx = g->hi[1][c], my = g->hi[2][c], E = g->hi[3][c];
         float inv = 1.0f / r;
         float vx = mx * inv, vy = my * inv, e = E * inv;
-        float T = (e - 0.5f * (vx * vx + vy * vy)) * g->rcv;
+        float T = fmaf(-0.5f, fmaf(vx, vx, vy * vy), e) * g->rcv;
         float a = sqrtf(g->gR * T);
         float P = (r * g->R) * T;
-        float U[4], f[4], z1, z2, z3, M;
+        float U[4], f[4], z1, z2, z3;
         U[0] = r; U[1] = mx; U[2] = my; U[3] = E;
-        M = vx / a;
-        z1 = 0.5f * (M + 1.0f); z2 = (0.5f * a) * (1.0f - M * M); z3 = 0.5f * (M - 1.0f);
-        f[0] = mx; f[1] = mx * vx + P; f[2] = mx * vy; f[3] = vx * (E + P);
-        for (v = 0; v < 4; v++) {
-            g->fp[v][c] = f[v] * z1 + U[v] * z2;
-            g->fm[v][c] = -f[v] * z3 - U[v] * z2;
-        }
-        M = vy / a;
-        z1 = 0.5f * (M + 1.0f); z2 = (0.5f * a) * (1.0f - M * M); z3 = 0.5f * (M - 1.0f);
-        f[0] = my; f[1] = my * vx; f[2] = my * vy + P; f[3] = vy * (E + P);
-        for (v = 0; v < 4; v++) {
-            g->hp[v][c] = f[v] * z1 + U[v] * z2;
-            g->hm[v][c] = -f[v] * z3 - U[v] * z2;
-        }
+        c32_z(vx / a, a, &z1, &z2, &z3);
+        f[0] = mx; f[1] = fmaf(mx, vx, P); f[2] = mx * vy; f[3] = vx * (E + P);
+        for (v = 0; v < 4; v++) c32_split(f[v], U[v], z1, z2, z3, &g->fp[v][c], &g->fm[v][c]);
+        c32_z(vy / a, a, &z1, &z2, &z3);
+        f[0] = my; f[1] = my * vx; f[2] = fmaf(my, vy, P); f[3] = vy * (E + P);
+        for (v = 0; v < 4; v++) c32_split(f[v], U[v], z1, z2, z3, &g->hp[v][c], &g->hm[v][c]);
     }
 }
 
@@ -263,12 +275,13 @@ static float c32_slope(const float *q, int c, int s, int stride, int len)
  *                 G_0 = fp_0 + fm_0            (outflow:    L = fp_0)
  *   wall, k = len G_len = fp_{len-1} + S*fp_{len-1}  /  fp_{len-1} + fm_{len-1}
  * so that u_s -= lam*(((fp-fm)+R)-L) [- 0.5*lam*(((dfp+dfm)-Rdf)-Ldf)] equals
- * u_s += t_s - t_{s+1} in exact arithmetic.
+ * u_s += t_s - t_{s+1} in exact arithmetic.  Second order evaluates t_k as
+ * fmaf(lam/2, d_k, lam*G_k) with d_k = dfp_{k-1} - dfm_k (0 at the walls).
  */
 static float c32_face(const c32grid *g, float **qp, float **qm, const float *sgn, int v, int c0, int k,
                       int stride, int len, float lam)
 {
-    float G;
+    float G, d = 0.0f;
     const float *p = qp[v], *m = qm[v];
     if (k == 0) {
         G = g->bc == ORC_BC_REFLECT ? m[c0] + sgn[v] * m[c0] : p[c0] + m[c0];
@@ -278,10 +291,18 @@ static float c32_face(const c32grid *g, float **qp, float **qm, const float *sgn
     } else {
         int cl = c0 + (k - 1) * stride, cr = cl + stride;
         G = p[cl] + m[cr];
-        if (g->order == 2)
-            G = G + 0.5f * (c32_slope(p, cl, k - 1, stride, len) - c32_slope(m, cr, k, stride, len));
+        if (g->order == 2) d = c32_slope(p, cl, k - 1, stride, len) - c32_slope(m, cr, k, stride, len);
     }
-    return lam * G;
+    /* second order: lam*(G + d/2) as one fused multiply-add (d = 0 at the walls) */
+    return g->order == 2 ? fmaf(0.5f * lam, d, lam * G) : lam * G;
+}
+
+/* Fast2Sum (Dekker): renormalise (a, b) with |b| below an ulp of a */
+static void fast_two_sum(float a, float b, float *s, float *e)
+{
+    float ss = a + b;
+    *e = b - (ss - a);
+    *s = ss;
 }
 
 static void c32_step(c32grid *g)
@@ -308,7 +329,7 @@ static void c32_step(c32grid *g)
                 two_sum(a, tb, &a, &e); b = b + e;
                 two_sum(a, -tt, &a, &e); b = b + e;
             }
-            two_sum(a, b, &g->hi[v][c], &g->lo[v][c]);
+            fast_two_sum(a, b, &g->hi[v][c], &g->lo[v][c]);
         }
     }
 }

@@ -88,14 +88,15 @@ struct RefArith {
     }
 };
 
-// FP32-compensated arithmetic: plain float, same formula shapes.
+// FP32-compensated arithmetic: float, with explicit fused multiply-adds where a
+// product feeds a sum (one rounding instead of two, and one instruction).
 struct C32Arith {
     __device__ static __forceinline__ void prim1(float u0, float u1, float u2, float rcv, float gR, float& vx,
                                                  float& T, float& a) {
         float inv = 1.0f / u0;
         vx = u1 * inv;
         float e = u2 * inv;
-        T = (e - 0.5f * (vx * vx)) * rcv;
+        T = __builtin_fmaf(-0.5f * vx, vx, e) * rcv;  // (e - vx^2/2) / cv
         a = sqrtf(gR * T);
     }
     __device__ static __forceinline__ void prim2(float u0, float u1, float u2, float u3, float rcv, float gR,
@@ -104,13 +105,15 @@ struct C32Arith {
         vx = u1 * inv;
         vy = u2 * inv;
         float e = u3 * inv;
-        T = (e - 0.5f * (vx * vx + vy * vy)) * rcv;
+        T = __builtin_fmaf(-0.5f, __builtin_fmaf(vx, vx, vy * vy), e) * rcv;
         a = sqrtf(gR * T);
     }
+    // z1 = (M+1)/2, z2 = (a/2)(1-M^2), z3 = (M-1)/2
     __device__ static __forceinline__ void zinv(float M, float a, float& z1, float& z2, float& z3) {
-        z1 = 0.5f * (M + 1.0f);
-        z2 = (0.5f * a) * (1.0f - M * M);
-        z3 = 0.5f * (M - 1.0f);
+        const float h = 0.5f * M;
+        z1 = h + 0.5f;
+        z2 = (0.5f * a) * __builtin_fmaf(-M, M, 1.0f);
+        z3 = h - 0.5f;
     }
     // minmod as the median of (l, r, 0): the smaller magnitude when the signs agree,
     // zero otherwise -- one v_med3_f32.  Unlike the reference's product test it also
@@ -127,11 +130,25 @@ __device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
     e = (a - (s - bb)) + (b - bb);
 }
 
+// Fast2Sum (Dekker 1971): s + e == a + b exactly when |a| >= |b| (the renormalisation
+// of a (hi, lo) pair, whose lo is below an ulp of hi)
+__device__ __forceinline__ void fast_two_sum(float a, float b, float& s, float& e) {
+    s = a + b;
+    e = b - (s - a);
+}
+
 // SHLL split of one flux component: F+ = F*Z1 + U*Z2, F- = -F*Z3 - U*Z2 (base_shll.c:149-157)
 template <class RT>
 __device__ __forceinline__ void split(RT f, RT u, RT z1, RT z2, RT z3, RT& fp, RT& fm) {
     fp = f * z1 + u * z2;
     fm = -f * z3 - u * z2;
+}
+
+// the same split in the compensated scheme's float arithmetic: U*Z2 once, two FMAs
+__device__ __forceinline__ void split_c32(float f, float u, float z1, float z2, float z3, float& fp, float& fm) {
+    const float uz = u * z2;
+    fp = __builtin_fmaf(f, z1, uz);
+    fm = __builtin_fmaf(-f, z3, -uz);
 }
 
 }  // namespace shll

@@ -51,6 +51,24 @@ __device__ __forceinline__ void make_window(const RT (&q)[C], RT (&W)[C + 4]) {
     }
 }
 
+// The wall rules as ghost fluxes in the windows: F+ of cell -1 and F- of cell n become
+// the values the reference's boundary cells use for their missing neighbour
+// (base_shll.c:93-110, 2nd_order_base_shll.c:212-256).  Reflective: F+(-1) = s*F-(0),
+// F-(n) = s*F+(n-1) with s = -1 for mass and energy, +1 for momentum; outflow:
+// F+(-1) = F+(0), F-(n) = F-(n-1).  Together with zero slopes on cells <= 0 and
+// >= n-1 this gives every owned cell the reference's boundary update.
+template <class RT, int C>
+__device__ __forceinline__ void wall_ghosts(RT (&FP)[C + 4], RT (&FM)[C + 4], int gl, int n, bool refl, RT sgn) {
+#pragma unroll
+    for (int i = 0; i + 1 < C + 4; ++i) {
+        const int c = gl - 2 + i;  // cell of window index i
+        const RT gp = refl ? sgn * FM[i + 1] : FP[i + 1];
+        const RT gm = refl ? sgn * FP[i] : FM[i];
+        FP[i] = c == -1 ? gp : FP[i];
+        FM[i + 1] = c + 1 == n ? gm : FM[i + 1];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // F64 / REF32: the reference association, cell by cell
 // ---------------------------------------------------------------------------
@@ -80,25 +98,19 @@ __device__ __forceinline__ void step_ref(typename Real<PREC>::T (&u)[3][C], cons
         RT FP[C + 4], FM[C + 4];
         make_window<RT, C>(fp[v], FP);
         make_window<RT, C>(fm[v], FM);
-        const RT sgn = v == 1 ? (RT)1 : (RT)-1;  // reflective wall signs (base_shll.c:95-110)
+        if constexpr (WALLS) wall_ghosts<RT, C>(FP, FM, gl, n, refl, v == 1 ? (RT)1 : (RT)-1);
 #pragma unroll
         for (int m = 0; m < C; ++m) {
             const int g = gl + m;
-            RT Lv = FP[m + 1], Rv = FM[m + 3];
-            if constexpr (WALLS) {
-                Lv = g == 0 ? (refl ? sgn * fm[v][m] : fp[v][m]) : Lv;
-                Rv = g == n - 1 ? (refl ? sgn * fp[v][m] : fm[v][m]) : Rv;
-            }
-            RT uu = u[v][m] - lam * (fp[v][m] - fm[v][m] + Rv - Lv);
+            // FL = F+ of the left neighbour, FR = F- of the right one (ghosts at the walls)
+            RT uu = u[v][m] - lam * (fp[v][m] - fm[v][m] + FM[m + 3] - FP[m + 1]);
             if constexpr (ORDER == 2) {
+                // own slopes, the left neighbour's F+ slope and the right neighbour's F-
+                // slope; every slope of a boundary or ghost cell is zero
                 RT dp = slope_at<A, WALLS>(FP, m, g, n);
                 RT dm = slope_at<A, WALLS>(FM, m, g, n);
                 RT Ldf = slope_at<A, WALLS>(FP, m - 1, g - 1, n);
                 RT Rdf = slope_at<A, WALLS>(FM, m + 1, g + 1, n);
-                if constexpr (WALLS) {
-                    Ldf = g == 0 ? (RT)0 : Ldf;
-                    Rdf = g == n - 1 ? (RT)0 : Rdf;
-                }
                 uu = (RT)((double)uu - 0.5 * (double)lam * (double)(dp + dm - Rdf - Ldf));
             }
             u[v][m] = uu;
@@ -112,7 +124,7 @@ __device__ __forceinline__ void step_ref(typename Real<PREC>::T (&u)[3][C], cons
 template <int ORDER, int C, bool WALLS>
 __device__ __forceinline__ void step_c32(float (&hi)[3][C], float (&lo)[3][C], const Inst& in, int gl) {
     using A = C32Arith;
-    const float rcv = in.k.c_rcv, gR = in.k.c_gR, lam = in.k.c_lamx;
+    const float rcv = in.k.c_rcv, gR = in.k.c_gR, lam = in.k.c_lamx, hlam = 0.5f * lam;
     const int n = in.nx;
     float fp[3][C], fm[3][C];
 #pragma unroll
@@ -121,10 +133,10 @@ __device__ __forceinline__ void step_c32(float (&hi)[3][C], float (&lo)[3][C], c
         A::prim1(hi[0][m], hi[1][m], hi[2][m], rcv, gR, vx, T, a);
         A::zinv(vx / a, a, z1, z2, z3);
         float P = hi[0][m] * T;  // R = 1
-        float f0 = hi[1][m], f1 = hi[1][m] * vx + P, f2 = vx * (hi[2][m] + P);
-        split<float>(f0, hi[0][m], z1, z2, z3, fp[0][m], fm[0][m]);
-        split<float>(f1, hi[1][m], z1, z2, z3, fp[1][m], fm[1][m]);
-        split<float>(f2, hi[2][m], z1, z2, z3, fp[2][m], fm[2][m]);
+        float f0 = hi[1][m], f1 = __builtin_fmaf(hi[1][m], vx, P), f2 = vx * (hi[2][m] + P);
+        split_c32(f0, hi[0][m], z1, z2, z3, fp[0][m], fm[0][m]);
+        split_c32(f1, hi[1][m], z1, z2, z3, fp[1][m], fm[1][m]);
+        split_c32(f2, hi[2][m], z1, z2, z3, fp[2][m], fm[2][m]);
     }
     const bool refl = in.bc == BC_REFLECT;
 #pragma unroll
@@ -132,19 +144,17 @@ __device__ __forceinline__ void step_c32(float (&hi)[3][C], float (&lo)[3][C], c
         float FP[C + 4], FM[C + 4], t[C + 1];
         make_window<float, C>(fp[v], FP);
         make_window<float, C>(fm[v], FM);
-        const float sgn = v == 1 ? 1.0f : -1.0f;
+        if constexpr (WALLS) wall_ghosts<float, C>(FP, FM, gl, n, refl, v == 1 ? 1.0f : -1.0f);
 #pragma unroll
         for (int f = 0; f <= C; ++f) {
             const int gf = gl + f;  // face gf lies between cells gf-1 and gf
-            float G = FP[f + 1] + FM[f + 2];
-            if constexpr (ORDER == 2)
-                G = G + 0.5f * (slope_at<A, WALLS>(FP, f - 1, gf - 1, n) - slope_at<A, WALLS>(FM, f, gf, n));
-            if constexpr (WALLS) {
-                float Gl = refl ? FM[f + 2] + sgn * FM[f + 2] : FP[f + 2] + FM[f + 2];
-                float Gr = refl ? FP[f + 1] + sgn * FP[f + 1] : FP[f + 1] + FM[f + 1];
-                G = gf == 0 ? Gl : (gf == n ? Gr : G);
+            const float G = FP[f + 1] + FM[f + 2];
+            if constexpr (ORDER == 2) {
+                const float d = slope_at<A, WALLS>(FP, f - 1, gf - 1, n) - slope_at<A, WALLS>(FM, f, gf, n);
+                t[f] = __builtin_fmaf(hlam, d, lam * G);  // lam*(G + d/2)
+            } else {
+                t[f] = lam * G;
             }
-            t[f] = lam * G;
         }
 #pragma unroll
         for (int m = 0; m < C; ++m) {
@@ -153,7 +163,7 @@ __device__ __forceinline__ void step_c32(float (&hi)[3][C], float (&lo)[3][C], c
             b = b + e;
             two_sum(a, -t[m + 1], a, e);
             b = b + e;
-            two_sum(a, b, hi[v][m], lo[v][m]);
+            fast_two_sum(a, b, hi[v][m], lo[v][m]);
         }
     }
 }

@@ -34,20 +34,32 @@ __device__ __forceinline__ void fluxes2(const RT (&U)[4], const Inst& in, RT (&x
         RefArith<RT>::prim2(U[0], U[1], U[2], U[3], cv, gR, vx, vy, T, a);
     }
     const RT P = U[0] * T;  // (rho*R)*T with R = 1 (base_shll_2d.c:24, :246)
-    // x direction (base_shll_2d.c:249-272)
-    if constexpr (PREC == PREC_C32) C32Arith::zinv(vx / a, a, z1, z2, z3);
-    else RefArith<RT>::zinv(vx / a, a, z1, z2, z3);
-    split<RT>(U[1], U[0], z1, z2, z3, xp[0], xm[0]);
-    split<RT>(U[1] * vx + P, U[1], z1, z2, z3, xp[1], xm[1]);
-    split<RT>(U[1] * vy, U[2], z1, z2, z3, xp[2], xm[2]);
-    split<RT>(vx * (U[3] + P), U[3], z1, z2, z3, xp[3], xm[3]);
-    // y direction (base_shll_2d.c:275-298)
-    if constexpr (PREC == PREC_C32) C32Arith::zinv(vy / a, a, z1, z2, z3);
-    else RefArith<RT>::zinv(vy / a, a, z1, z2, z3);
-    split<RT>(U[2], U[0], z1, z2, z3, yp[0], ym[0]);
-    split<RT>(U[2] * vx, U[1], z1, z2, z3, yp[1], ym[1]);
-    split<RT>(U[2] * vy + P, U[2], z1, z2, z3, yp[2], ym[2]);
-    split<RT>(vy * (U[3] + P), U[3], z1, z2, z3, yp[3], ym[3]);
+    if constexpr (PREC == PREC_C32) {
+        // the compensated scheme's float form: fused multiply-adds (split_c32)
+        C32Arith::zinv(vx / a, a, z1, z2, z3);
+        split_c32(U[1], U[0], z1, z2, z3, xp[0], xm[0]);
+        split_c32(__builtin_fmaf(U[1], vx, P), U[1], z1, z2, z3, xp[1], xm[1]);
+        split_c32(U[1] * vy, U[2], z1, z2, z3, xp[2], xm[2]);
+        split_c32(vx * (U[3] + P), U[3], z1, z2, z3, xp[3], xm[3]);
+        C32Arith::zinv(vy / a, a, z1, z2, z3);
+        split_c32(U[2], U[0], z1, z2, z3, yp[0], ym[0]);
+        split_c32(U[2] * vx, U[1], z1, z2, z3, yp[1], ym[1]);
+        split_c32(__builtin_fmaf(U[2], vy, P), U[2], z1, z2, z3, yp[2], ym[2]);
+        split_c32(vy * (U[3] + P), U[3], z1, z2, z3, yp[3], ym[3]);
+    } else {
+        // x direction (base_shll_2d.c:249-272)
+        RefArith<RT>::zinv(vx / a, a, z1, z2, z3);
+        split<RT>(U[1], U[0], z1, z2, z3, xp[0], xm[0]);
+        split<RT>(U[1] * vx + P, U[1], z1, z2, z3, xp[1], xm[1]);
+        split<RT>(U[1] * vy, U[2], z1, z2, z3, xp[2], xm[2]);
+        split<RT>(vx * (U[3] + P), U[3], z1, z2, z3, xp[3], xm[3]);
+        // y direction (base_shll_2d.c:275-298)
+        RefArith<RT>::zinv(vy / a, a, z1, z2, z3);
+        split<RT>(U[2], U[0], z1, z2, z3, yp[0], ym[0]);
+        split<RT>(U[2] * vx, U[1], z1, z2, z3, yp[1], ym[1]);
+        split<RT>(U[2] * vy + P, U[2], z1, z2, z3, yp[2], ym[2]);
+        split<RT>(vy * (U[3] + P), U[3], z1, z2, z3, yp[3], ym[3]);
+    }
 }
 
 // reflective wall signs: x walls (-,+,-,-), y walls (-,-,+,-)
@@ -166,21 +178,28 @@ __global__ __launch_bounds__(64 * WPB) void k_step2d(StepArgs a) {
                 }
                 if (jown && c < it.ix0 + it.iw) o0p[v * P + o] = uu;
             } else {
-                // face terms (regrouped reference update), TwoSum into (hi, lo)
+                // face terms t = lam*(G + d/2) (regrouped reference update; d = slope
+                // difference, zero at walls and in first order), TwoSum into (hi, lo)
                 const RT fpl = xp[ORDER - 1][v], fmr = xm[ORDER + 1][v];
                 RT Gl = fpl + fmc, Gr = fpc + fmr;
                 RT Gb = hpl + hmc;
-                if constexpr (ORDER == 2) {
-                    Gl = Gl + 0.5f * (dfpl - dfm);
-                    Gr = Gr + 0.5f * (dfp - dfmr);
-                    Gb = Gb + 0.5f * (dhpl - dhm);
-                }
                 Gl = c == 0 ? (refl ? fmc + sx * fmc : fpc + fmc) : Gl;
                 Gr = c == nx - 1 ? (refl ? fpc + sx * fpc : fpc + fmc) : Gr;
                 Gb = j == 0 ? (refl ? hmc + sy * hmc : hpc + hmc) : Gb;
-                const RT tl = lamx * Gl, tr = lamx * Gr, tb = lamy * Gb;
+                RT tl = lamx * Gl, tr = lamx * Gr, tb = lamy * Gb;
+                if constexpr (ORDER == 2) {
+                    const RT dl = c == 0 ? (RT)0 : dfpl - dfm;
+                    const RT dr = c == nx - 1 ? (RT)0 : dfp - dfmr;
+                    const RT db = j == 0 ? (RT)0 : dhpl - dhm;
+                    tl = __builtin_fmaf(0.5f * lamx, dl, tl);
+                    tr = __builtin_fmaf(0.5f * lamx, dr, tr);
+                    tb = __builtin_fmaf(0.5f * lamy, db, tb);
+                }
                 RT tt = from_right(tb);
-                tt = j == ny - 1 ? lamy * (refl ? hpc + sy * hpc : hpc + hmc) : tt;
+                if constexpr (ORDER == 2)
+                    tt = j == ny - 1 ? __builtin_fmaf(0.5f * lamy, (RT)0, lamy * (refl ? hpc + sy * hpc : hpc + hmc)) : tt;
+                else
+                    tt = j == ny - 1 ? lamy * (refl ? hpc + sy * hpc : hpc + hmc) : tt;
                 RT aa = su[0][v], bb = sl[0][v], e;
                 two_sum(aa, tl, aa, e);
                 bb = bb + e;
@@ -190,7 +209,7 @@ __global__ __launch_bounds__(64 * WPB) void k_step2d(StepArgs a) {
                 bb = bb + e;
                 two_sum(aa, -tt, aa, e);
                 bb = bb + e;
-                two_sum(aa, bb, aa, e);
+                fast_two_sum(aa, bb, aa, e);
                 if (jown && c < it.ix0 + it.iw) {
                     o0p[v * P + o] = aa;
                     o1p[v * P + o] = e;
