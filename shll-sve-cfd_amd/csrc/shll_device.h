@@ -88,25 +88,56 @@ struct RefArith {
     }
 };
 
+// ---------------------------------------------------------------------------
+// Correctly rounded float division and square root for operands in the range a
+// flow state produces.  These are the instruction sequences the compiler emits for
+// IEEE '/' and sqrtf (-fhip-fp32-correctly-rounded-divide-sqrt) minus their range
+// handling: the exponent pre-scaling of v_div_scale / v_div_fmas and the special-
+// value pass of v_div_fixup, and sqrt's 2^32 pre-scale below 2^-96 and class
+// pass-through.  For finite nonzero operands with |a|, |b| and |a/b| well inside
+// [2^-96, 2^96] (and x >= 2^-96 for the root) no scaling applies and the results
+// are those of '/' and sqrtf bit for bit; zero, infinite and NaN operands still
+// give a zero, infinite or NaN result (the sign of a zero quotient may differ).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float div_rn(float a, float b) {
+    float r = __builtin_amdgcn_rcpf(b);
+    const float e = __builtin_fmaf(-b, r, 1.0f);
+    r = __builtin_fmaf(e, r, r);           // 1/b to ~1/2 ulp
+    float q = a * r;
+    const float d = __builtin_fmaf(-b, q, a);
+    q = __builtin_fmaf(d, r, q);
+    const float d2 = __builtin_fmaf(-b, q, a);
+    return __builtin_fmaf(d2, r, q);       // one exact-residual correction
+}
+__device__ __forceinline__ float sqrt_rn(float x) {
+    float s = __builtin_amdgcn_sqrtf(x);   // within 1 ulp
+    const float sdn = __int_as_float(__float_as_int(s) - 1);
+    const float sup = __int_as_float(__float_as_int(s) + 1);
+    const float rdn = __builtin_fmaf(-sdn, s, x);
+    const float rup = __builtin_fmaf(-sup, s, x);
+    s = rdn <= 0.0f ? sdn : s;
+    return rup > 0.0f ? sup : s;
+}
+
 // FP32-compensated arithmetic: float, with explicit fused multiply-adds where a
 // product feeds a sum (one rounding instead of two, and one instruction).
 struct C32Arith {
     __device__ static __forceinline__ void prim1(float u0, float u1, float u2, float rcv, float gR, float& vx,
                                                  float& T, float& a) {
-        float inv = 1.0f / u0;
+        float inv = div_rn(1.0f, u0);
         vx = u1 * inv;
         float e = u2 * inv;
         T = __builtin_fmaf(-0.5f * vx, vx, e) * rcv;  // (e - vx^2/2) / cv
-        a = sqrtf(gR * T);
+        a = sqrt_rn(gR * T);
     }
     __device__ static __forceinline__ void prim2(float u0, float u1, float u2, float u3, float rcv, float gR,
                                                  float& vx, float& vy, float& T, float& a) {
-        float inv = 1.0f / u0;
+        float inv = div_rn(1.0f, u0);
         vx = u1 * inv;
         vy = u2 * inv;
         float e = u3 * inv;
         T = __builtin_fmaf(-0.5f, __builtin_fmaf(vx, vx, vy * vy), e) * rcv;
-        a = sqrtf(gR * T);
+        a = sqrt_rn(gR * T);
     }
     // z1 = (M+1)/2, z2 = (a/2)(1-M^2), z3 = (M-1)/2
     __device__ static __forceinline__ void zinv(float M, float a, float& z1, float& z2, float& z3) {

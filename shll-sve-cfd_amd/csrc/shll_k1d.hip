@@ -131,7 +131,7 @@ __device__ __forceinline__ void step_c32(float (&hi)[3][C], float (&lo)[3][C], c
     for (int m = 0; m < C; ++m) {
         float vx, T, a, z1, z2, z3;
         A::prim1(hi[0][m], hi[1][m], hi[2][m], rcv, gR, vx, T, a);
-        A::zinv(vx / a, a, z1, z2, z3);
+        A::zinv(div_rn(vx, a), a, z1, z2, z3);
         float P = hi[0][m] * T;  // R = 1
         float f0 = hi[1][m], f1 = __builtin_fmaf(hi[1][m], vx, P), f2 = vx * (hi[2][m] + P);
         split_c32(f0, hi[0][m], z1, z2, z3, fp[0][m], fm[0][m]);
