@@ -152,20 +152,22 @@ struct Launch {
     int rows = 64;      // rows per 2D item
 };
 
-// 1D tiling of one instance of nx cells: segments of 64*C cells, halo order*k.
+// 1D tiling of one instance of nx cells: segments of 64*C cells, halo order*k.  The
+// segment that holds a wall also holds that wall's ghost cell (-1 or nx), so the
+// first segment starts at cell -1; the last one always reaches past cell nx.
 static bool tile1d(int32_t inst, int64_t nx, int order, int C, int k, std::vector<Item1>& out) {
     const int64_t S = 64LL * C;
-    if (nx <= S) {
-        out.push_back(Item1{inst, 0, 0, (int32_t)nx});
+    if (nx + 2 <= S) {
+        out.push_back(Item1{inst, -1, 0, (int32_t)nx});
         return true;
     }
     const int64_t H = (int64_t)order * k;
     const int64_t W = S - 2 * H;
-    if (S - H <= 0 || W <= 0) return false;
+    if (S - 1 - H <= 0 || W <= 0) return false;
     int64_t x0 = 0;
     {
-        const int64_t w = std::min<int64_t>(nx, S - H);
-        out.push_back(Item1{inst, 0, 0, (int32_t)w});
+        const int64_t w = std::min<int64_t>(nx, S - 1 - H);
+        out.push_back(Item1{inst, -1, 0, (int32_t)w});
         x0 = w;
     }
     while (x0 < nx) {

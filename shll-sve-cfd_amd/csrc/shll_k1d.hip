@@ -22,14 +22,18 @@ template <int PREC> struct Real { using T = double; };
 template <> struct Real<PREC_REF32> { using T = float; };
 template <> struct Real<PREC_C32> { using T = float; };
 
+// Wall modes of a wavefront's segment (a wave-uniform choice, see k_step1d):
+//   WM_NONE   the segment is at least 3 cells from both ends: no boundary rule applies
+//   WM_GHOST  ghost cells -1 and n (ghost_cells) carry the walls
+//   WM_RULES  ghost cells plus the explicit zero-slope rule (reflective second order)
+enum { WM_NONE = 0, WM_GHOST = 1, WM_RULES = 2 };
+
 // slope of a split-flux window at position p (window index p+2) of global cell gg;
-// zero on the two boundary cells (2nd_order_base_shll.c:227-256)
-// WALLS = false: the wavefront's segment is at least 3 cells from both ends, so no
-// boundary rule can apply (wave-uniform choice, see k_step1d)
-template <class A, bool WALLS, class RT, int NW>
+// with WM_RULES zero on the two boundary cells and beyond (2nd_order_base_shll.c:227-256)
+template <class A, int WM, class RT, int NW>
 __device__ __forceinline__ RT slope_at(const RT (&W)[NW], int p, int gg, int n) {
     RT s = A::mm(W[p + 2] - W[p + 1], W[p + 3] - W[p + 2]);
-    if constexpr (!WALLS) return s;
+    if constexpr (WM != WM_RULES) return s;
     return (gg <= 0 || gg >= n - 1) ? (RT)0 : s;
 }
 
@@ -51,28 +55,44 @@ __device__ __forceinline__ void make_window(const RT (&q)[C], RT (&W)[C + 4]) {
     }
 }
 
-// The wall rules as ghost fluxes in the windows: F+ of cell -1 and F- of cell n become
-// the values the reference's boundary cells use for their missing neighbour
-// (base_shll.c:93-110, 2nd_order_base_shll.c:212-256).  Reflective: F+(-1) = s*F-(0),
-// F-(n) = s*F+(n-1) with s = -1 for mass and energy, +1 for momentum; outflow:
-// F+(-1) = F+(0), F-(n) = F-(n-1).  Together with zero slopes on cells <= 0 and
-// >= n-1 this gives every owned cell the reference's boundary update.
-template <class RT, int C>
-__device__ __forceinline__ void wall_ghosts(RT (&FP)[C + 4], RT (&FM)[C + 4], int gl, int n, bool refl, RT sgn) {
+// Walls as ghost cells: before each step the state of cell -1 becomes that of cell 0
+// and the state of cell n that of cell n-1 -- copied for outflow walls, mirrored
+// (momentum negated) for reflective ones.  Every operation of the split flux is odd
+// or even in the velocity, so the ghosts' fluxes are bitwise the values the
+// reference's boundary rules use for the missing neighbour: F+(-1) = F+(0) and
+// F-(n) = F-(n-1) for outflow (2nd_order_base_shll.c:212-256), F+(-1) = s*F-(0) and
+// F-(n) = s*F+(n-1) with s = (-1, +1, -1) for reflection (base_shll.c:93-110).  With
+// copied ghosts every boundary slope also comes out zero through the limiter (one of
+// its differences is exactly 0), as the reference sets them; reflective second order
+// needs the explicit rule (WM_RULES).  Cells beyond the ghosts only feed the ghosts'
+// own updates, which are replaced again before the next step.
+template <int C, class RT>
+__device__ __forceinline__ void ghost_cells(RT (&q)[3][C], int gl, int n, bool refl) {
 #pragma unroll
-    for (int i = 0; i + 1 < C + 4; ++i) {
-        const int c = gl - 2 + i;  // cell of window index i
-        const RT gp = refl ? sgn * FM[i + 1] : FP[i + 1];
-        const RT gm = refl ? sgn * FP[i] : FM[i];
-        FP[i] = c == -1 ? gp : FP[i];
-        FM[i + 1] = c + 1 == n ? gm : FM[i + 1];
+    for (int v = 0; v < 3; ++v) {
+        const RT next = from_right(q[v][0]);     // cell gl+C
+        const RT prev = from_left(q[v][C - 1]);  // cell gl-1
+        RT out[C];
+#pragma unroll
+        for (int m = 0; m < C; ++m) {
+            RT right = m + 1 < C ? q[v][m + 1] : next;  // cell g+1
+            RT left = m > 0 ? q[v][m - 1] : prev;       // cell g-1
+            if (v == 1) {
+                right = refl ? -right : right;
+                left = refl ? -left : left;
+            }
+            const int g = gl + m;
+            out[m] = g == -1 ? right : (g == n ? left : q[v][m]);
+        }
+#pragma unroll
+        for (int m = 0; m < C; ++m) q[v][m] = out[m];
     }
 }
 
 // ---------------------------------------------------------------------------
 // F64 / REF32: the reference association, cell by cell
 // ---------------------------------------------------------------------------
-template <int PREC, int ORDER, int C, bool WALLS>
+template <int PREC, int ORDER, int C, int WM>
 __device__ __forceinline__ void step_ref(typename Real<PREC>::T (&u)[3][C], const Inst& in, int gl) {
     using RT = typename Real<PREC>::T;
     using A = RefArith<RT>;
@@ -80,6 +100,8 @@ __device__ __forceinline__ void step_ref(typename Real<PREC>::T (&u)[3][C], cons
     const RT gR = PREC == PREC_F64 ? (RT)in.k.d_gR : (RT)in.k.f_gR;
     const RT lam = PREC == PREC_F64 ? (RT)in.k.d_lamx : (RT)in.k.f_lamx;
     const int n = in.nx;
+    const bool refl = in.bc == BC_REFLECT;
+    if constexpr (WM != WM_NONE) ghost_cells<C>(u, gl, n, refl);
     RT fp[3][C], fm[3][C];
 #pragma unroll
     for (int m = 0; m < C; ++m) {
@@ -92,13 +114,11 @@ __device__ __forceinline__ void step_ref(typename Real<PREC>::T (&u)[3][C], cons
         split<RT>(f1, u[1][m], z1, z2, z3, fp[1][m], fm[1][m]);
         split<RT>(f2, u[2][m], z1, z2, z3, fp[2][m], fm[2][m]);
     }
-    const bool refl = in.bc == BC_REFLECT;
 #pragma unroll
     for (int v = 0; v < 3; ++v) {
         RT FP[C + 4], FM[C + 4];
         make_window<RT, C>(fp[v], FP);
         make_window<RT, C>(fm[v], FM);
-        if constexpr (WALLS) wall_ghosts<RT, C>(FP, FM, gl, n, refl, v == 1 ? (RT)1 : (RT)-1);
 #pragma unroll
         for (int m = 0; m < C; ++m) {
             const int g = gl + m;
@@ -107,10 +127,10 @@ __device__ __forceinline__ void step_ref(typename Real<PREC>::T (&u)[3][C], cons
             if constexpr (ORDER == 2) {
                 // own slopes, the left neighbour's F+ slope and the right neighbour's F-
                 // slope; every slope of a boundary or ghost cell is zero
-                RT dp = slope_at<A, WALLS>(FP, m, g, n);
-                RT dm = slope_at<A, WALLS>(FM, m, g, n);
-                RT Ldf = slope_at<A, WALLS>(FP, m - 1, g - 1, n);
-                RT Rdf = slope_at<A, WALLS>(FM, m + 1, g + 1, n);
+                RT dp = slope_at<A, WM>(FP, m, g, n);
+                RT dm = slope_at<A, WM>(FM, m, g, n);
+                RT Ldf = slope_at<A, WM>(FP, m - 1, g - 1, n);
+                RT Rdf = slope_at<A, WM>(FM, m + 1, g + 1, n);
                 uu = (RT)((double)uu - 0.5 * (double)lam * (double)(dp + dm - Rdf - Ldf));
             }
             u[v][m] = uu;
@@ -121,11 +141,12 @@ __device__ __forceinline__ void step_ref(typename Real<PREC>::T (&u)[3][C], cons
 // ---------------------------------------------------------------------------
 // C32: float fluxes of the hi word, face terms t = lam*G added with TwoSum
 // ---------------------------------------------------------------------------
-template <int ORDER, int C, bool WALLS>
+template <int ORDER, int C, int WM>
 __device__ __forceinline__ void step_c32(float (&hi)[3][C], float (&lo)[3][C], const Inst& in, int gl) {
     using A = C32Arith;
     const float rcv = in.k.c_rcv, gR = in.k.c_gR, lam = in.k.c_lamx, hlam = 0.5f * lam;
     const int n = in.nx;
+    if constexpr (WM != WM_NONE) ghost_cells<C>(hi, gl, n, in.bc == BC_REFLECT);
     float fp[3][C], fm[3][C];
 #pragma unroll
     for (int m = 0; m < C; ++m) {
@@ -138,19 +159,17 @@ __device__ __forceinline__ void step_c32(float (&hi)[3][C], float (&lo)[3][C], c
         split_c32(f1, hi[1][m], z1, z2, z3, fp[1][m], fm[1][m]);
         split_c32(f2, hi[2][m], z1, z2, z3, fp[2][m], fm[2][m]);
     }
-    const bool refl = in.bc == BC_REFLECT;
 #pragma unroll
     for (int v = 0; v < 3; ++v) {
         float FP[C + 4], FM[C + 4], t[C + 1];
         make_window<float, C>(fp[v], FP);
         make_window<float, C>(fm[v], FM);
-        if constexpr (WALLS) wall_ghosts<float, C>(FP, FM, gl, n, refl, v == 1 ? 1.0f : -1.0f);
 #pragma unroll
         for (int f = 0; f <= C; ++f) {
             const int gf = gl + f;  // face gf lies between cells gf-1 and gf
             const float G = FP[f + 1] + FM[f + 2];
             if constexpr (ORDER == 2) {
-                const float d = slope_at<A, WALLS>(FP, f - 1, gf - 1, n) - slope_at<A, WALLS>(FM, f, gf, n);
+                const float d = slope_at<A, WM>(FP, f - 1, gf - 1, n) - slope_at<A, WM>(FM, f, gf, n);
                 t[f] = __builtin_fmaf(hlam, d, lam * G);  // lam*(G + d/2)
             } else {
                 t[f] = lam * G;
@@ -164,6 +183,48 @@ __device__ __forceinline__ void step_c32(float (&hi)[3][C], float (&lo)[3][C], c
             two_sum(a, -t[m + 1], a, e);
             b = b + e;
             fast_two_sum(a, b, hi[v][m], lo[v][m]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// ns steps of one segment from registers, then the owned cells back to HBM.  Each
+// wall mode gets its own copy of loop and store, so the loop-carried state never
+// has to be moved between the copies' registers.
+// ---------------------------------------------------------------------------
+template <int ORDER, int C, int WM>
+__device__ __forceinline__ void advance_c32(float (&hi)[3][C], float (&lo)[3][C], const Inst& in, const Item1& it,
+                                            int gl, int ns, const StepArgs& a) {
+    for (int s = 0; s < ns; ++s) step_c32<ORDER, C, WM>(hi, lo, in, gl);
+    float* __restrict__ oh = (float*)a.out0 + in.off;
+    float* __restrict__ ol = (float*)a.out1 + in.off;
+    const int64_t P = a.plane;
+#pragma unroll
+    for (int m = 0; m < C; ++m) {
+        const int g = gl + m;
+        if (g >= it.x0 && g < it.x0 + it.w) {
+#pragma unroll
+            for (int v = 0; v < 3; ++v) {
+                oh[v * P + g] = hi[v][m];
+                ol[v * P + g] = lo[v][m];
+            }
+        }
+    }
+}
+
+template <int PREC, int ORDER, int C, int WM>
+__device__ __forceinline__ void advance_ref(typename Real<PREC>::T (&u)[3][C], const Inst& in, const Item1& it,
+                                            int gl, int ns, const StepArgs& a) {
+    using RT = typename Real<PREC>::T;
+    for (int s = 0; s < ns; ++s) step_ref<PREC, ORDER, C, WM>(u, in, gl);
+    RT* __restrict__ ou = (RT*)a.out0 + in.off;
+    const int64_t P = a.plane;
+#pragma unroll
+    for (int m = 0; m < C; ++m) {
+        const int g = gl + m;
+        if (g >= it.x0 && g < it.x0 + it.w) {
+#pragma unroll
+            for (int v = 0; v < 3; ++v) ou[v * P + g] = u[v][m];
         }
     }
 }
@@ -188,6 +249,8 @@ __global__ __launch_bounds__(64 * WPB) void k_step1d(StepArgs a) {
     const int64_t P = a.plane;
     // wall rules can only apply if the segment reaches within 3 cells of an end
     const bool walls = it.g0 <= 2 || it.g0 + 64 * C >= n - 3;
+    const int wm = !walls ? WM_NONE : (ORDER == 2 && in.bc == BC_REFLECT ? WM_RULES : WM_GHOST);
+    constexpr int WR = ORDER == 2 ? WM_RULES : WM_GHOST;
 
     if constexpr (PREC == PREC_C32) {
         const float* __restrict__ ih = (const float*)a.in0 + in.off;
@@ -203,23 +266,9 @@ __global__ __launch_bounds__(64 * WPB) void k_step1d(StepArgs a) {
                 lo[v][m] = ok ? il[v * P + g] : 0.0f;
             }
         }
-        if (walls)
-            for (int s = 0; s < ns; ++s) step_c32<ORDER, C, true>(hi, lo, in, gl);
-        else
-            for (int s = 0; s < ns; ++s) step_c32<ORDER, C, false>(hi, lo, in, gl);
-        float* __restrict__ oh = (float*)a.out0 + in.off;
-        float* __restrict__ ol = (float*)a.out1 + in.off;
-#pragma unroll
-        for (int m = 0; m < C; ++m) {
-            const int g = gl + m;
-            if (g >= it.x0 && g < it.x0 + it.w) {
-#pragma unroll
-                for (int v = 0; v < 3; ++v) {
-                    oh[v * P + g] = hi[v][m];
-                    ol[v * P + g] = lo[v][m];
-                }
-            }
-        }
+        if (wm == WM_NONE) advance_c32<ORDER, C, WM_NONE>(hi, lo, in, it, gl, ns, a);
+        else if (ORDER == 1 || wm == WM_GHOST) advance_c32<ORDER, C, WM_GHOST>(hi, lo, in, it, gl, ns, a);
+        else advance_c32<ORDER, C, WR>(hi, lo, in, it, gl, ns, a);
     } else {
         const RT* __restrict__ iu = (const RT*)a.in0 + in.off;
         RT u[3][C];
@@ -230,19 +279,9 @@ __global__ __launch_bounds__(64 * WPB) void k_step1d(StepArgs a) {
 #pragma unroll
             for (int v = 0; v < 3; ++v) u[v][m] = ok ? iu[v * P + g] : (RT)(v == 0 ? 1.0 : (v == 1 ? 0.0 : 2.5));
         }
-        if (walls)
-            for (int s = 0; s < ns; ++s) step_ref<PREC, ORDER, C, true>(u, in, gl);
-        else
-            for (int s = 0; s < ns; ++s) step_ref<PREC, ORDER, C, false>(u, in, gl);
-        RT* __restrict__ ou = (RT*)a.out0 + in.off;
-#pragma unroll
-        for (int m = 0; m < C; ++m) {
-            const int g = gl + m;
-            if (g >= it.x0 && g < it.x0 + it.w) {
-#pragma unroll
-                for (int v = 0; v < 3; ++v) ou[v * P + g] = u[v][m];
-            }
-        }
+        if (wm == WM_NONE) advance_ref<PREC, ORDER, C, WM_NONE>(u, in, it, gl, ns, a);
+        else if (ORDER == 1 || wm == WM_GHOST) advance_ref<PREC, ORDER, C, WM_GHOST>(u, in, it, gl, ns, a);
+        else advance_ref<PREC, ORDER, C, WR>(u, in, it, gl, ns, a);
     }
 }
 
