@@ -103,7 +103,7 @@ def run_batch(S, pbs, prims, steps, launch, K, W, progress=0):
     if launch:
         b.set_launch(*launch)
     else:
-        b.autotune(64)
+        b.autotune(2048)
     if W > 0:
         b.advance(W)
         b.reset()
@@ -247,7 +247,9 @@ def tube_errors(S, b, pbs, prims, steps, K):
     an FP64 batch on every 16th tube in size order.  The reference scheme's fixed time
     step DT = 0.125*DX assumes wave speeds near R+1; hot, strongly expanding tubes exceed
     (|u|+a)*DT/DX = 0.5 and the forward-Euler second-order scheme then goes non-finite in
-    FP64 as well, so those tubes are counted and left out of the error statistics."""
+    FP64 as well, so those tubes are counted and left out of the error statistics.  Tubes
+    on the edge of that instability can blow up in one precision and not the other: the
+    ones non-finite only in FP32-compensated are counted separately and also left out."""
     import numpy as np
     order = sorted(range(len(pbs)), key=lambda k: pbs[k].nx)
     sample = [k for k in order[::16] if steps[k] <= K]
@@ -259,11 +261,15 @@ def tube_errors(S, b, pbs, prims, steps, K):
     bad64 = ref.check_state()
     bad32 = b.check_state()
     rows = []
+    c32_only = 0
     for q, k in enumerate(sample):
         if bad64[q] > 0:
             continue
+        if bad32[k] > 0:
+            c32_only += 1
+            continue
         e = b.compare(k, ref, q)
-        rows.append((pbs[k].nx, e["l1_rho"], e["linf_rho"], bad32[k] > 0))
+        rows.append((pbs[k].nx, e["l1_rho"], e["linf_rho"]))
     ref.close()
     l1 = np.array([r[1] for r in rows])
     n = np.array([r[0] for r in rows])
@@ -277,6 +283,7 @@ def tube_errors(S, b, pbs, prims, steps, K):
     return {
         "judge": "FP64 kernel, every 16th tube in size order (%d tubes)" % len(sample),
         "tubes_nonfinite_fp64_in_sample": int((bad64 > 0).sum()),
+        "tubes_nonfinite_c32_only_in_sample": c32_only,
         "tubes_nonfinite_c32_all": int((bad32 > 0).sum()),
         "compared_tubes": len(rows),
         "l1_rho_median": float(np.median(l1)) if len(l1) else None,

@@ -149,7 +149,7 @@ struct Launch {
     int k = 8;          // steps per launch (1D)
     int wpb = 4;        // waves per block
     int graph = 128;    // steps per graph (0: no graph)
-    int rows = 64;      // rows per 2D item
+    int rows = 16;      // rows per 2D item
 };
 
 // 1D tiling of one instance of nx cells: segments of 64*C cells, halo order*k.  The
@@ -162,11 +162,14 @@ static bool tile1d(int32_t inst, int64_t nx, int order, int C, int k, std::vecto
         return true;
     }
     const int64_t H = (int64_t)order * k;
-    const int64_t W = S - 2 * H;
-    if (S - 1 - H <= 0 || W <= 0) return false;
+    if (S - 1 - H <= 0 || S - 2 * H <= 0) return false;
+    // owned ranges start on 32-cell boundaries (128-byte lines of every plane, whose
+    // instances start 64-cell aligned), so no line is written by two segments
+    auto lines = [](int64_t w) { return w >= 32 ? w / 32 * 32 : w; };
+    const int64_t W = lines(S - 2 * H);
     int64_t x0 = 0;
     {
-        const int64_t w = std::min<int64_t>(nx, S - 1 - H);
+        const int64_t w = std::min<int64_t>(nx, lines(S - 1 - H));
         out.push_back(Item1{inst, -1, 0, (int32_t)w});
         x0 = w;
     }
@@ -693,10 +696,12 @@ static shll_status engine_create_stream(Engine* E) {
 
 static void default_launch(Engine* E, int64_t max_cells) {
     if (E->dim == 2) {
+        // 16-row items: a 4096^2 domain in 64-row items leaves ~4 wavefronts per SIMD,
+        // too few to cover the row loads; 16 rows measured 10-30% faster
         E->L.C = 1;
         E->L.k = 1;
         E->L.wpb = 4;
-        E->L.rows = 64;
+        E->L.rows = 16;
         E->L.graph = 64;
         return;
     }
@@ -1043,11 +1048,8 @@ static shll_status engine_autotune(Engine* E, int32_t trial_steps, int64_t shape
         for (int C : {1, 2, 4, 8})
             for (int k : {1, 2, 4, 8, 16, 32, 64})
                 for (int w : {1, 4}) {
-                    // batches: 256-thread workgroups and at least 8 steps per launch keep the
-                    // dispatch count of a whole run near 10^4 (rocprofv3 kernel tracing of
-                    // 2.6e4 single-wave dispatches crashed in the tool); the tuner measured
-                    // the excluded shapes within a few percent of the kept ones
-                    if (E->batch && (w != 4 || k < 8)) continue;
+                    // batches: fewer than 4 steps per launch only multiplies the launches
+                    if (E->batch && k < 4) continue;
                     Launch L = start;
                     L.C = C; L.k = k; L.wpb = w;
                     L.graph = std::max(128, 2 * k);

@@ -7,10 +7,10 @@
 //   PREC_C32   : float arithmetic, (hi, lo) float state, conservative face update
 //                with TwoSum (the FP32-compensated form)
 //
-// The library is compiled with -ffp-contract=off: every + - * / below is one
-// IEEE-754 operation in source order, '/' and sqrt are correctly rounded
-// (-fhip-fp32-correctly-rounded-divide-sqrt, IEEE f64 div/sqrt), so the F64 and
-// REF32 kernels reproduce the CPU results bit for bit.
+// The library is compiled with -ffp-contract=off: every + - * below is one IEEE-754
+// operation in source order (fused multiply-adds only where written), and division
+// and square root are correctly rounded (div_rn, sqrt_rn), so the F64 and REF32
+// kernels reproduce the CPU results bit for bit.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -51,6 +51,76 @@ __device__ __forceinline__ double from_right(double x) {
 __device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
 
 // ---------------------------------------------------------------------------
+// Correctly rounded division and square root.  The fast paths (div_fast, sqrt_fast)
+// are the instruction sequences the compiler emits for IEEE '/', sqrtf and sqrt
+// (-fhip-fp32-correctly-rounded-divide-sqrt; FP64 is always IEEE) minus their range
+// handling -- the exponent pre-scaling of v_div_scale / v_div_fmas, the special-value
+// pass of v_div_fixup, the square roots' pre-scaling of tiny radicands and their
+// pass-through of 0 and inf.  With operands and results of moderate size (no
+// intermediate leaves the normal range) they give the same bits as the operators.
+// div_rn and sqrt_rn check the result or radicand (a quotient of at least 2^-64 in
+// float or 2^-900 in double, a radicand of at least 2^-90 / 2^-760) and hand every
+// other case -- zero, tiny, denormal, NaN -- to the operators themselves, so they
+// are always bitwise '/' and sqrt.  The compensated scheme calls the fast paths
+// directly: its divisors (density, sound speed) and radicand (gamma*R*T) are of order
+// one, and its one possibly tiny quotient, the Mach number M, only enters as M/2 +-
+// 1/2 and 1 - M^2, which absorb any last-place difference of a tiny M.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float div_fast(float a, float b) {
+    float r = __builtin_amdgcn_rcpf(b);
+    const float e = __builtin_fmaf(-b, r, 1.0f);
+    r = __builtin_fmaf(e, r, r);           // 1/b to ~1/2 ulp
+    float q = a * r;
+    const float d = __builtin_fmaf(-b, q, a);
+    q = __builtin_fmaf(d, r, q);
+    const float d2 = __builtin_fmaf(-b, q, a);
+    return __builtin_fmaf(d2, r, q);       // one exact-residual correction
+}
+__device__ __forceinline__ double div_fast(double a, double b) {
+    double r = __builtin_amdgcn_rcp(b);
+    double e = __builtin_fma(-b, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-b, r, 1.0);
+    r = __builtin_fma(r, e, r);            // two Newton steps on 1/b
+    const double q = a * r;
+    const double d = __builtin_fma(-b, q, a);
+    return __builtin_fma(d, r, q);
+}
+__device__ __forceinline__ float sqrt_fast(float x) {
+    float s = __builtin_amdgcn_sqrtf(x);   // within 1 ulp
+    const float sdn = __int_as_float(__float_as_int(s) - 1);
+    const float sup = __int_as_float(__float_as_int(s) + 1);
+    const float rdn = __builtin_fmaf(-sdn, s, x);
+    const float rup = __builtin_fmaf(-sup, s, x);
+    s = rdn <= 0.0f ? sdn : s;
+    return rup > 0.0f ? sup : s;
+}
+__device__ __forceinline__ double sqrt_fast(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = y * 0.5;         // sqrt(x) and 1/(2 sqrt(x)) estimates
+    const double r = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, r, g);
+    double d = __builtin_fma(-g, g, x);
+    h = __builtin_fma(h, r, h);
+    g = __builtin_fma(d, h, g);
+    d = __builtin_fma(-g, g, x);
+    return __builtin_fma(d, h, g);
+}
+__device__ __forceinline__ float div_rn(float a, float b) {
+    float q = div_fast(a, b);
+    if (__builtin_expect(!(fabsf(q) >= 0x1p-64f), 0)) q = a / b;
+    return q;
+}
+__device__ __forceinline__ double div_rn(double a, double b) {
+    double q = div_fast(a, b);
+    if (__builtin_expect(!(fabs(q) >= 0x1p-900), 0)) q = a / b;
+    return q;
+}
+__device__ __forceinline__ double sqrt_rn(double x) {
+    return __builtin_expect(!(x >= 0x1p-760), 0) ? sqrt(x) : sqrt_fast(x);
+}
+
+// ---------------------------------------------------------------------------
 // Reference association (F64 and REF32).  RT is the storage/arithmetic type;
 // (double) conversions sit where the reference's double literals and its double
 // sqrt() put them (base_shll.c:135-139, :172-175; base_shll_2d.c:313-317).
@@ -62,18 +132,18 @@ struct RefArith {
     // (the 2D files' association, used by the derived 1D second-order scheme).
     __device__ static __forceinline__ void prim1(RT u0, RT u1, RT u2, int ke, RT cv, RT gR, RT& vx, RT& T,
                                                  RT& a) {
-        vx = u1 / u0;
+        vx = div_rn(u1, u0);
         double kin = ke == 0 ? (0.5 * (double)vx) * (double)vx : 0.5 * (double)(vx * vx);
-        T = (RT)(((double)(u2 / u0) - kin) / (double)cv);
-        a = (RT)sqrt((double)(gR * T));
+        T = (RT)div_rn((double)div_rn(u2, u0) - kin, (double)cv);
+        a = (RT)sqrt_rn((double)(gR * T));
     }
     // Compute_P_from_U, 2D (base_shll_2d.c:313-317)
     __device__ static __forceinline__ void prim2(RT u0, RT u1, RT u2, RT u3, RT cv, RT gR, RT& vx, RT& vy,
                                                  RT& T, RT& a) {
-        vx = u1 / u0;
-        vy = u2 / u0;
-        T = (RT)(((double)(u3 / u0) - 0.5 * (double)(vx * vx + vy * vy)) / (double)cv);
-        a = (RT)sqrt((double)(gR * T));
+        vx = div_rn(u1, u0);
+        vy = div_rn(u2, u0);
+        T = (RT)div_rn((double)div_rn(u3, u0) - 0.5 * (double)(vx * vx + vy * vy), (double)cv);
+        a = (RT)sqrt_rn((double)(gR * T));
     }
     // Z invariants (base_shll.c:137-139)
     __device__ static __forceinline__ void zinv(RT M, RT a, RT& z1, RT& z2, RT& z3) {
@@ -88,56 +158,25 @@ struct RefArith {
     }
 };
 
-// ---------------------------------------------------------------------------
-// Correctly rounded float division and square root for operands in the range a
-// flow state produces.  These are the instruction sequences the compiler emits for
-// IEEE '/' and sqrtf (-fhip-fp32-correctly-rounded-divide-sqrt) minus their range
-// handling: the exponent pre-scaling of v_div_scale / v_div_fmas and the special-
-// value pass of v_div_fixup, and sqrt's 2^32 pre-scale below 2^-96 and class
-// pass-through.  For finite nonzero operands with |a|, |b| and |a/b| well inside
-// [2^-96, 2^96] (and x >= 2^-96 for the root) no scaling applies and the results
-// are those of '/' and sqrtf bit for bit; zero, infinite and NaN operands still
-// give a zero, infinite or NaN result (the sign of a zero quotient may differ).
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ float div_rn(float a, float b) {
-    float r = __builtin_amdgcn_rcpf(b);
-    const float e = __builtin_fmaf(-b, r, 1.0f);
-    r = __builtin_fmaf(e, r, r);           // 1/b to ~1/2 ulp
-    float q = a * r;
-    const float d = __builtin_fmaf(-b, q, a);
-    q = __builtin_fmaf(d, r, q);
-    const float d2 = __builtin_fmaf(-b, q, a);
-    return __builtin_fmaf(d2, r, q);       // one exact-residual correction
-}
-__device__ __forceinline__ float sqrt_rn(float x) {
-    float s = __builtin_amdgcn_sqrtf(x);   // within 1 ulp
-    const float sdn = __int_as_float(__float_as_int(s) - 1);
-    const float sup = __int_as_float(__float_as_int(s) + 1);
-    const float rdn = __builtin_fmaf(-sdn, s, x);
-    const float rup = __builtin_fmaf(-sup, s, x);
-    s = rdn <= 0.0f ? sdn : s;
-    return rup > 0.0f ? sup : s;
-}
-
 // FP32-compensated arithmetic: float, with explicit fused multiply-adds where a
 // product feeds a sum (one rounding instead of two, and one instruction).
 struct C32Arith {
     __device__ static __forceinline__ void prim1(float u0, float u1, float u2, float rcv, float gR, float& vx,
                                                  float& T, float& a) {
-        float inv = div_rn(1.0f, u0);
+        float inv = div_fast(1.0f, u0);
         vx = u1 * inv;
         float e = u2 * inv;
         T = __builtin_fmaf(-0.5f * vx, vx, e) * rcv;  // (e - vx^2/2) / cv
-        a = sqrt_rn(gR * T);
+        a = sqrt_fast(gR * T);
     }
     __device__ static __forceinline__ void prim2(float u0, float u1, float u2, float u3, float rcv, float gR,
                                                  float& vx, float& vy, float& T, float& a) {
-        float inv = div_rn(1.0f, u0);
+        float inv = div_fast(1.0f, u0);
         vx = u1 * inv;
         vy = u2 * inv;
         float e = u3 * inv;
         T = __builtin_fmaf(-0.5f, __builtin_fmaf(vx, vx, vy * vy), e) * rcv;
-        a = sqrt_rn(gR * T);
+        a = sqrt_fast(gR * T);
     }
     // z1 = (M+1)/2, z2 = (a/2)(1-M^2), z3 = (M-1)/2
     __device__ static __forceinline__ void zinv(float M, float a, float& z1, float& z2, float& z3) {

@@ -107,7 +107,7 @@ __device__ __forceinline__ void step_ref(typename Real<PREC>::T (&u)[3][C], cons
     for (int m = 0; m < C; ++m) {
         RT vx, T, a, z1, z2, z3;
         A::prim1(u[0][m], u[1][m], u[2][m], in.ke, cv, gR, vx, T, a);
-        A::zinv(vx / a, a, z1, z2, z3);
+        A::zinv(div_rn(vx, a), a, z1, z2, z3);
         RT P = u[0][m] * T;  // (rho*R)*T with R = 1 (base_shll.c:17, :141)
         RT f0 = u[1][m], f1 = u[1][m] * vx + P, f2 = vx * (u[2][m] + P);
         split<RT>(f0, u[0][m], z1, z2, z3, fp[0][m], fm[0][m]);
@@ -152,7 +152,7 @@ __device__ __forceinline__ void step_c32(float (&hi)[3][C], float (&lo)[3][C], c
     for (int m = 0; m < C; ++m) {
         float vx, T, a, z1, z2, z3;
         A::prim1(hi[0][m], hi[1][m], hi[2][m], rcv, gR, vx, T, a);
-        A::zinv(div_rn(vx, a), a, z1, z2, z3);
+        A::zinv(div_fast(vx, a), a, z1, z2, z3);
         float P = hi[0][m] * T;  // R = 1
         float f0 = hi[1][m], f1 = __builtin_fmaf(hi[1][m], vx, P), f2 = vx * (hi[2][m] + P);
         split_c32(f0, hi[0][m], z1, z2, z3, fp[0][m], fm[0][m]);

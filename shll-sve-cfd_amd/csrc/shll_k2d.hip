@@ -36,25 +36,25 @@ __device__ __forceinline__ void fluxes2(const RT (&U)[4], const Inst& in, RT (&x
     const RT P = U[0] * T;  // (rho*R)*T with R = 1 (base_shll_2d.c:24, :246)
     if constexpr (PREC == PREC_C32) {
         // the compensated scheme's float form: fused multiply-adds (split_c32)
-        C32Arith::zinv(div_rn(vx, a), a, z1, z2, z3);
+        C32Arith::zinv(div_fast(vx, a), a, z1, z2, z3);
         split_c32(U[1], U[0], z1, z2, z3, xp[0], xm[0]);
         split_c32(__builtin_fmaf(U[1], vx, P), U[1], z1, z2, z3, xp[1], xm[1]);
         split_c32(U[1] * vy, U[2], z1, z2, z3, xp[2], xm[2]);
         split_c32(vx * (U[3] + P), U[3], z1, z2, z3, xp[3], xm[3]);
-        C32Arith::zinv(div_rn(vy, a), a, z1, z2, z3);
+        C32Arith::zinv(div_fast(vy, a), a, z1, z2, z3);
         split_c32(U[2], U[0], z1, z2, z3, yp[0], ym[0]);
         split_c32(U[2] * vx, U[1], z1, z2, z3, yp[1], ym[1]);
         split_c32(__builtin_fmaf(U[2], vy, P), U[2], z1, z2, z3, yp[2], ym[2]);
         split_c32(vy * (U[3] + P), U[3], z1, z2, z3, yp[3], ym[3]);
     } else {
         // x direction (base_shll_2d.c:249-272)
-        RefArith<RT>::zinv(vx / a, a, z1, z2, z3);
+        RefArith<RT>::zinv(div_rn(vx, a), a, z1, z2, z3);
         split<RT>(U[1], U[0], z1, z2, z3, xp[0], xm[0]);
         split<RT>(U[1] * vx + P, U[1], z1, z2, z3, xp[1], xm[1]);
         split<RT>(U[1] * vy, U[2], z1, z2, z3, xp[2], xm[2]);
         split<RT>(vx * (U[3] + P), U[3], z1, z2, z3, xp[3], xm[3]);
         // y direction (base_shll_2d.c:275-298)
-        RefArith<RT>::zinv(vy / a, a, z1, z2, z3);
+        RefArith<RT>::zinv(div_rn(vy, a), a, z1, z2, z3);
         split<RT>(U[2], U[0], z1, z2, z3, yp[0], ym[0]);
         split<RT>(U[2] * vx, U[1], z1, z2, z3, yp[1], ym[1]);
         split<RT>(U[2] * vy + P, U[2], z1, z2, z3, yp[2], ym[2]);

@@ -21,10 +21,12 @@ def main():
     os.environ["SHLL_TUNE_CACHE"] = out
     import shll_amd as S
 
-    # batched FP32-compensated second-order tubes (the benchmark batch)
+    # batched FP32-compensated second-order tubes (the benchmark batch); the trial
+    # window (2048 steps, every tube of >= 1280 cells still running) weights the tube
+    # sizes roughly as the whole run does
     pbs, prims, steps = S.tube_batch(20261202, 4096, precision=S.C32)
     b = S.Batch(pbs, prims, steps)
-    b.autotune(64)
+    b.autotune(2048)
     print("batch c32 o2:", b.get_launch(), flush=True)
     b.close()
     # single instances: 1D first order 1024 cells, 1D second order 65536 cells
