@@ -5,14 +5,16 @@ Headline workload (the batched FP32-compensated form the throughput target is se
   4096 independent second-order shock tubes (seed 20261202), N log-uniform in
   [256, 65536] cells, rho_L/rho_R in [2, 20], T_L/T_R in [0.5, 2], diaphragm in
   [0.3, 0.7], u = 0, outflow walls, each run to t = 0.2 (its own step count from the
-  reference's float time loop), FP32 fluxes with a compensated (hi, lo) FP32 state,
-  one persistent launch per k steps over an (instance, tile) work list, HIP graphs.
+  reference's float time loop), FP32 fluxes with a compensated (hi, lo) FP32 state
+  updated by one Kahan step per cell and variable (SHLL_C32K), one launch per k steps
+  over an (instance, tile) work list, HIP graphs.
 A "step" is one time step of every unfinished tube.  The default run times the whole
 job (every tube to t = 0.2).  value = total cell-updates / wall time of the timed region.
 
-Also reported (extra): the single-instance configurations (1D first order 1024 cells,
-1D second order 65536 cells in FP64 and FP32-compensated, 2D first order 4096^2, and 64
-uneven 2D domains), each run in full with its error against the FP64 kernel.
+Also reported (extra): the same batch with the exactly conservative TwoSum update
+(SHLL_C32), the single-instance configurations (1D first order 1024 cells, 1D second
+order 65536 cells and 2D first order 4096^2 in FP64 and both FP32-compensated forms, and
+64 uneven 2D domains), each run in full with its error against the FP64 kernel.
 
 cpu_baseline: the oracle's reference-precision port of the reference C loop (unfused
 passes, gcc -O3) on one host core, on a stratified sample of the same tubes.
@@ -48,17 +50,17 @@ def bytes_per_update(dim, prec):
     (2 * nvar * bytes per value; FP32-compensated stores hi and lo)."""
     import shll_amd as S
     nv = 3 if dim == 1 else 4
-    s = {S.F64: 8, S.REF32: 4, S.C32: 8}[prec]
+    s = {S.F64: 8, S.REF32: 4, S.C32: 8, S.C32K: 8}[prec]
     return 2 * nv * s
 
 
-def traffic_per_update(shape):
+def traffic_per_update(shape, form="c32k"):
     """Calibrated HBM bytes per cell-update of the batch step kernel, from the committed
     rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE scaled by a 1 GiB copy), when the
     summary was taken with the same launch shape (cells/lane, steps/launch, waves/block);
     else None."""
     import re
-    path = os.path.join(ROOT, "profiles", "k_step1d_c32_o2_tubes4096_hbm_traffic.md")
+    path = os.path.join(ROOT, "profiles", "k_step1d_%s_o2_tubes4096_hbm_traffic.md" % form)
     if not os.path.exists(path):
         return None
     m = re.search(r"hbm_bytes_per_cell_update:\s*([0-9.eE+-]+)\s*\(launch shape (\d+),(\d+),(\d+)\)",
@@ -173,11 +175,11 @@ def extra_single(S):
         dt = time.perf_counter() - t0
         s2.close()
     out["c0_1d_o1_f64_N1024"] = {"steps": n, "seconds": dt, "cell_updates_per_s": 1024 * n / dt}
-    # 1D second order, 65536 cells, t = 0.2, FP64 and FP32-compensated
+    # 1D second order, 65536 cells, t = 0.2, FP64 and both FP32-compensated forms
     n1 = 65536
     ic = S.ic_tube(n1)
     sol = {}
-    for name, prec in (("f64", S.F64), ("c32", S.C32)):
+    for name, prec in (("f64", S.F64), ("c32", S.C32), ("c32k", S.C32K)):
         pb = S.problem(1, 2, n1, precision=prec)
         s = S.Solver(pb, ic)
         s.autotune(2000)
@@ -192,15 +194,15 @@ def extra_single(S):
         sol[name] = s
         out["c1_1d_o2_%s_N65536" % name] = {"steps": n, "seconds": dt, "cell_updates_per_s": n1 * n / dt,
                                            "launch": list(s.get_launch())}
-    e = sol["c32"].compare(sol["f64"])
-    out["c1_1d_o2_c32_N65536"]["error_vs_f64"] = e
+    for name in ("c32", "c32k"):
+        out["c1_1d_o2_%s_N65536" % name]["error_vs_f64"] = sol[name].compare(sol["f64"])
     for s in sol.values():
         s.close()
     # 2D first order, 4096^2, 2000 steps, FP64 and FP32-compensated
     n2 = 4096
     ic = S.ic_circle(n2, n2)
     sol = {}
-    for name, prec in (("f64", S.F64), ("c32", S.C32)):
+    for name, prec in (("f64", S.F64), ("c32", S.C32), ("c32k", S.C32K)):
         s = S.Solver(S.problem(2, 1, n2, n2, precision=prec), ic)
         s.step(20)
         s.close()
@@ -211,46 +213,44 @@ def extra_single(S):
         sol[name] = s
         out["c2_2d_o1_%s_4096sq" % name] = {"steps": 2000, "seconds": dt,
                                            "cell_updates_per_s": n2 * n2 * 2000 / dt}
-    out["c2_2d_o1_c32_4096sq"]["error_vs_f64"] = sol["c32"].compare(sol["f64"])
+    for name in ("c32", "c32k"):
+        out["c2_2d_o1_%s_4096sq" % name]["error_vs_f64"] = sol[name].compare(sol["f64"])
     for s in sol.values():
         s.close()
-    # 64 uneven 2D domains, first order, 1000 steps, FP32-compensated;
-    # the FP64 single-domain kernel judges accuracy on 4 of them
-    pbs, prims = S.domain_batch(4242, 64, precision=S.C32)
-    b = S.Batch(pbs, prims, [1000] * 64)
-    b.advance(10)
-    b.reset()
-    t0 = time.perf_counter()
-    b.run()
-    dt = time.perf_counter() - t0
+    # 64 uneven 2D domains, first order, 1000 steps, both FP32-compensated
+    # forms; the FP64 batch of the same domains judges accuracy
+    pbs, prims = S.domain_batch(4242, 64, precision=S.C32K)
     cells = sum(p.nx * p.ny for p in pbs)
     ref = S.Batch([S.problem(2, 1, p.nx, p.ny, precision=S.F64, lx=p.lx, ly=p.ly) for p in pbs], prims,
                   [1000] * 64)
     t1 = time.perf_counter()
     ref.run()
     dt64 = time.perf_counter() - t1
-    errs = [b.compare(k, ref)["l1_rho"] / (pbs[k].lx * pbs[k].ly) for k in range(64)]
-    nonfinite = int((ref.check_state() > 0).sum()) + int((b.check_state() > 0).sum())
+    bad64 = int((ref.check_state() > 0).sum())
+    for name, prec in (("c32k", S.C32K), ("c32", S.C32)):
+        pbp = [S.problem(2, 1, p.nx, p.ny, precision=prec, lx=p.lx, ly=p.ly) for p in pbs]
+        b = S.Batch(pbp, prims, [1000] * 64)
+        b.advance(10)
+        b.reset()
+        t0 = time.perf_counter()
+        b.run()
+        dt = time.perf_counter() - t0
+        errs = [b.compare(k, ref)["l1_rho"] / (pbs[k].lx * pbs[k].ly) for k in range(64)]
+        nonfinite = bad64 + int((b.check_state() > 0).sum())
+        b.close()
+        out["c4_2d_o1_%s_64domains" % name] = {"steps": 1000, "cells": cells, "seconds": dt,
+                                               "cell_updates_per_s": cells * 1000 / dt,
+                                               "f64_batch_cell_updates_per_s": cells * 1000 / dt64,
+                                               "l1_rho_vs_f64_per_unit_area_max": max(errs),
+                                               "l1_rho_vs_f64_per_unit_area_median": sorted(errs)[32],
+                                               "nonfinite_domains": nonfinite}
     ref.close()
-    b.close()
-    out["c4_2d_o1_c32_64domains"] = {"steps": 1000, "cells": cells, "seconds": dt,
-                                     "cell_updates_per_s": cells * 1000 / dt,
-                                     "f64_batch_cell_updates_per_s": cells * 1000 / dt64,
-                                     "l1_rho_vs_f64_per_unit_area_max": max(errs),
-                                     "l1_rho_vs_f64_per_unit_area_median": sorted(errs)[32],
-                                     "nonfinite_domains": nonfinite}
     return out
 
 
-def tube_errors(S, b, pbs, prims, steps, K):
-    """Accuracy of the FP32-compensated batch against the FP64 kernel (the judge), run as
-    an FP64 batch on every 16th tube in size order.  The reference scheme's fixed time
-    step DT = 0.125*DX assumes wave speeds near R+1; hot, strongly expanding tubes exceed
-    (|u|+a)*DT/DX = 0.5 and the forward-Euler second-order scheme then goes non-finite in
-    FP64 as well, so those tubes are counted and left out of the error statistics.  Tubes
-    on the edge of that instability can blow up in one precision and not the other: the
-    ones non-finite only in FP32-compensated are counted separately and also left out."""
-    import numpy as np
+def make_judge(S, pbs, steps, prims, K):
+    """The FP64 kernel run as a batch on every 16th tube in size order (tubes that finish
+    within K steps), the accuracy judge of the FP32-compensated batches."""
     order = sorted(range(len(pbs)), key=lambda k: pbs[k].nx)
     sample = [k for k in order[::16] if steps[k] <= K]
     if not sample:
@@ -258,7 +258,21 @@ def tube_errors(S, b, pbs, prims, steps, K):
     ref = S.Batch([S.problem(1, 2, pbs[k].nx, precision=S.F64) for k in sample], [prims[k] for k in sample],
                   [steps[k] for k in sample])
     ref.run()
-    bad64 = ref.check_state()
+    return ref, sample, ref.check_state()
+
+
+def tube_errors(S, b, pbs, judge):
+    """Accuracy of an FP32-compensated batch against the FP64 judge.  The reference scheme's
+    fixed time step DT = 0.125*DX assumes wave speeds near R+1; hot, strongly expanding
+    tubes exceed (|u|+a)*DT/DX = 0.5 and the forward-Euler second-order scheme then goes
+    non-finite in FP64 as well, so those tubes are counted and left out of the error
+    statistics.  Tubes on the edge of that instability can blow up in one precision and
+    not the other: the ones non-finite only in FP32-compensated are counted separately and
+    also left out."""
+    import numpy as np
+    if judge is None:
+        return None
+    ref, sample, bad64 = judge
     bad32 = b.check_state()
     rows = []
     c32_only = 0
@@ -270,7 +284,6 @@ def tube_errors(S, b, pbs, prims, steps, K):
             continue
         e = b.compare(k, ref, q)
         rows.append((pbs[k].nx, e["l1_rho"], e["linf_rho"]))
-    ref.close()
     l1 = np.array([r[1] for r in rows])
     n = np.array([r[0] for r in rows])
     bands = {}
@@ -307,6 +320,8 @@ def main():
     ap.add_argument("--launch", default="", help="C,k,waves,graph (default: autotune)")
     ap.add_argument("--calib-copy", type=int, default=0, help="bytes of a device copy to run first (profiling)")
     ap.add_argument("--autotune", action="store_true", help="re-run the autotuner instead of its recorded choices")
+    ap.add_argument("--form", default="c32k", choices=["c32k", "c32"],
+                    help="compensated update of the timed batch: Kahan (c32k) or exact TwoSum (c32)")
     args = ap.parse_args()
     if not args.autotune:
         os.environ.setdefault("SHLL_TUNE_CACHE", TUNE_CACHE)
@@ -317,8 +332,12 @@ def main():
 
     if args.calib_copy > 0:
         calibration_copy(args.calib_copy)
+    form = S.PREC_NAMES[args.form]
+    other = S.C32 if form == S.C32K else S.C32K
+    names = {S.C32: "FP32 fluxes + (hi,lo) FP32 state, exact TwoSum face update",
+             S.C32K: "FP32 fluxes + (hi,lo) FP32 state, Kahan-compensated update"}
     t0 = time.perf_counter()
-    pbs, prims, steps = S.tube_batch(SEED, NTUBES, precision=S.C32)
+    pbs, prims, steps = S.tube_batch(SEED, NTUBES, precision=form)
     log("generated %d tubes in %.2fs, %d cells, max steps %d" % (len(pbs), time.perf_counter() - t0,
                                                                  sum(p.nx for p in pbs), max(steps)))
     K = args.steps if args.steps > 0 else max(steps)
@@ -326,13 +345,26 @@ def main():
     b, wall, avg_ms, launches, ev_ms = run_batch(S, pbs, prims, steps, launch, K, args.warmup, args.progress)
     upd = updates_done(pbs, steps, K)
     value = upd / wall
-    B = bytes_per_update(1, S.C32)
+    B = bytes_per_update(1, form)
     achieved = B * upd / (ev_ms * 1e-3) / 1e9 if ev_ms > 0 else 0.0
     lau = list(b.get_launch())
     log("timed %d steps: %.3e cell-updates in %.3f s (events %.1f ms, %d launches)" % (K, upd, wall, ev_ms, launches))
-    errs = None if args.no_accuracy else tube_errors(S, b, pbs, prims, steps, K)
+    judge = None if args.no_accuracy else make_judge(S, pbs, steps, prims, K)
+    errs = tube_errors(S, b, pbs, judge)
     b.close()
-    tpu = traffic_per_update(lau)
+    alt = None
+    if not args.no_extra:
+        # the same batch with the other compensated update, its own tuned shape
+        pbo = [S.problem(1, 2, p.nx, precision=other) for p in pbs]
+        bo, wo, ao, lo_, eo = run_batch(S, pbo, prims, steps, None, K, args.warmup, args.progress)
+        alt = {"form": names[other], "precision": "c32" if other == S.C32 else "c32k",
+               "cell_updates_per_s": upd / wo, "avg_launch_ms": ao, "launches": lo_,
+               "launch_shape": list(bo.get_launch()), "accuracy": tube_errors(S, bo, pbo, judge)}
+        log("other form: %.3e cell-updates/s" % (upd / wo))
+        bo.close()
+    if judge is not None:
+        judge[0].close()
+    tpu = traffic_per_update(lau, args.form)
     tr = tpu * upd / launches if tpu is not None and launches > 0 else None
     result = {
         "metric": METRIC,
@@ -348,7 +380,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": "batched 1D second-order SHLL shock tubes, FP32 fluxes + compensated FP32 (hi,lo) state",
+            "workload": "batched 1D second-order SHLL shock tubes, " + names[form],
             "tubes": NTUBES, "seed": SEED, "cells": int(sum(p.nx for p in pbs)),
             "N_range": [256, 65536], "t_end": 0.2, "max_steps": int(max(steps)),
             "cell_updates_timed": int(upd), "launches": launches, "parallelism": "1 GPU",
@@ -362,14 +394,14 @@ def main():
             "unit": "GB/s",
             "frac": achieved / PEAK_HBM_GBS,
             "traffic": tr,
-            "kernel": "shll::k_step1d<2, 2, %d, %d> (FP32-compensated, second order, %d cells/lane, %d waves/block)" % (
-                lau[0], lau[2], lau[0], lau[2]),
+            "kernel": "shll::k_step1d<%d, 2, %d, %d> (FP32-compensated %s, second order, %d cells/lane, %d waves/block)" % (
+                form, lau[0], lau[2], "Kahan" if form == S.C32K else "TwoSum", lau[0], lau[2]),
             "algorithmic_bytes_per_cell_update": B,
             "avg_launch_ms": avg_ms,
             "note": "achieved = 48 B x cell-updates / HIP-event time of the launches; temporal blocking "
                     "(k steps per launch from registers) lets it exceed the HBM peak. traffic = HBM bytes per "
                     "launch of the timed run: calibrated rocprofv3 FETCH_SIZE+WRITE_SIZE per cell-update "
-                    "(profiles/k_step1d_c32_o2_tubes4096_hbm_traffic.md) x cell-updates per launch",
+                    "(profiles/k_step1d_%s_o2_tubes4096_hbm_traffic.md) x cell-updates per launch" % args.form,
         },
         "accuracy": errs,
     }
@@ -378,7 +410,8 @@ def main():
     else:
         result["cpu_baseline"] = None
     if not args.no_extra:
-        result["extra"] = extra_single(S)
+        result["extra"] = {"c3_other_update": alt}
+        result["extra"].update(extra_single(S))
     print(json.dumps(result))
 
 

@@ -12,7 +12,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 
-F64, REF32, C32 = 0, 1, 2
+F64, REF32, C32, C32K = 0, 1, 2, 3
 BC_REFLECT, BC_OUTFLOW = 0, 1
 
 
@@ -53,7 +53,7 @@ def lib():
         L.orc_steps_to_time.restype = ctypes.c_int64
         L.orc_constants.argtypes = [P, ctypes.c_int, dp]
         L.orc_run.argtypes = [P, ctypes.c_int, dp, ctypes.c_int64, dp, dp, ctypes.c_int, dp]
-        L.orc_run_c32_raw.argtypes = [P, dp, ctypes.c_int64, fp, fp]
+        L.orc_run_c32_raw.argtypes = [P, ctypes.c_int, dp, ctypes.c_int64, fp, fp]
         L.orc_stage.argtypes = [P, ctypes.c_int, dp, dp, dp, dp, dp, dp, dp, dp]
         L.orc_minmod.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double]
         L.orc_minmod.restype = ctypes.c_double
@@ -102,11 +102,12 @@ def run(pb, mode, prim, nsteps, nthreads=1, timed=False):
     return po, co
 
 
-def run_c32_raw(pb, prim, nsteps):
+def run_c32_raw(pb, prim, nsteps, mode=C32):
+    """Raw (hi, lo) float planes of a C32 or C32K run."""
     prim = np.ascontiguousarray(prim, dtype=np.float64).reshape(nvar(pb), ncells(pb))
     hi = np.empty(prim.shape, dtype=np.float32)
     lo = np.empty(prim.shape, dtype=np.float32)
-    rc = lib().orc_run_c32_raw(ctypes.byref(pb), _dp(prim), int(nsteps),
+    rc = lib().orc_run_c32_raw(ctypes.byref(pb), int(mode), _dp(prim), int(nsteps),
                                hi.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
                                lo.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
     if rc != 0:

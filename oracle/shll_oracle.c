@@ -96,7 +96,7 @@ int64_t orc_steps_to_time(const orc_problem *pb, float t_end)
 /* conserves them.                                                            */
 /* ------------------------------------------------------------------------- */
 typedef struct {
-    int nx, ny, n, nv, dim, order, bc;
+    int nx, ny, n, nv, dim, order, bc, kahan;
     float R, gR, rcv, lamx, lamy, hlamx, hlamy;
     double cv64, gR64;
     float *hi[4], *lo[4], *fp[4], *fm[4], *hp[4], *hm[4];
@@ -321,11 +321,23 @@ static void c32_step(c32grid *g)
             float a = g->hi[v][c], b = g->lo[v][c], e;
             float tl = c32_face(g, g->fp, g->fm, g->dim == 1 ? sx1_c32 : sx2_c32, v, cx0, i, sx, g->nx, g->lamx);
             float tr = c32_face(g, g->fp, g->fm, g->dim == 1 ? sx1_c32 : sx2_c32, v, cx0, i + 1, sx, g->nx, g->lamx);
+            float tb = 0.0f, tt = 0.0f;
+            if (g->dim == 2) {
+                tb = c32_face(g, g->hp, g->hm, sy2_c32, v, cy0, j, 1, g->ny, g->lamy);
+                tt = c32_face(g, g->hp, g->hm, sy2_c32, v, cy0, j + 1, 1, g->ny, g->lamy);
+            }
+            if (g->kahan) {
+                /* Kahan (compensated summation): lo is the part of the value not yet in
+                 * hi; the cell's net face term y joins it and hi takes what fits */
+                float y = g->dim == 1 ? (tl - tr) + b : ((tl - tr) + (tb - tt)) + b;
+                float s = a + y;
+                g->lo[v][c] = y - (s - a);
+                g->hi[v][c] = s;
+                continue;
+            }
             two_sum(a, tl, &a, &e); b = b + e;
             two_sum(a, -tr, &a, &e); b = b + e;
             if (g->dim == 2) {
-                float tb = c32_face(g, g->hp, g->hm, sy2_c32, v, cy0, j, 1, g->ny, g->lamy);
-                float tt = c32_face(g, g->hp, g->hm, sy2_c32, v, cy0, j + 1, 1, g->ny, g->lamy);
                 two_sum(a, tb, &a, &e); b = b + e;
                 two_sum(a, -tt, &a, &e); b = b + e;
             }
@@ -348,13 +360,14 @@ static void c32_output(const c32grid *g, double *prim_out, double *cons_out)
     }
 }
 
-static int c32_run(const orc_problem *pb, const double *prim_in, int64_t nsteps, double *prim_out,
+static int c32_run(const orc_problem *pb, int kahan, const double *prim_in, int64_t nsteps, double *prim_out,
                    double *cons_out, float *hi_out, float *lo_out, double *elapsed)
 {
     c32grid g;
     int64_t s;
     double t0;
     if (c32_alloc(&g, pb)) return -2;
+    g.kahan = kahan;
     c32_init(&g, prim_in);
     t0 = orc_now();
     for (s = 0; s < nsteps; s++) c32_step(&g);
@@ -379,16 +392,17 @@ int orc_run(const orc_problem *pb, int mode, const double *prim_in, int64_t nste
     switch (mode) {
     case ORC_REF32: return run_r32(pb, prim_in, nsteps, prim_out, cons_out, nthreads, elapsed_s);
     case ORC_F64: return run_f64(pb, prim_in, nsteps, prim_out, cons_out, nthreads, elapsed_s);
-    case ORC_C32: return c32_run(pb, prim_in, nsteps, prim_out, cons_out, NULL, NULL, elapsed_s);
+    case ORC_C32: return c32_run(pb, 0, prim_in, nsteps, prim_out, cons_out, NULL, NULL, elapsed_s);
+    case ORC_C32K: return c32_run(pb, 1, prim_in, nsteps, prim_out, cons_out, NULL, NULL, elapsed_s);
     default: return -1;
     }
 }
 
-int orc_run_c32_raw(const orc_problem *pb, const double *prim_in, int64_t nsteps,
+int orc_run_c32_raw(const orc_problem *pb, int mode, const double *prim_in, int64_t nsteps,
                     float *hi_out, float *lo_out)
 {
-    if (orc_check(pb) || !prim_in || nsteps < 0) return -1;
-    return c32_run(pb, prim_in, nsteps, NULL, NULL, hi_out, lo_out, NULL);
+    if (orc_check(pb) || !prim_in || nsteps < 0 || (mode != ORC_C32 && mode != ORC_C32K)) return -1;
+    return c32_run(pb, mode == ORC_C32K, prim_in, nsteps, NULL, NULL, hi_out, lo_out, NULL);
 }
 
 void orc_constants(const orc_problem *pb, int mode, double out[6])
@@ -438,7 +452,7 @@ int orc_stage(const orc_problem *pb, int mode, const double *prim_in, double *co
         STAGE_BODY(_r32);
     } else if (mode == ORC_F64) {
         STAGE_BODY(_f64);
-    } else if (mode == ORC_C32) {
+    } else if (mode == ORC_C32 || mode == ORC_C32K) {
         c32grid g;
         int v, c;
         if (c32_alloc(&g, pb)) return -2;
@@ -466,7 +480,7 @@ int orc_stage(const orc_problem *pb, int mode, const double *prim_in, double *co
 double orc_minmod(int mode, double l, double r)
 {
     if (mode == ORC_REF32) return (double)mm_r32((float)l, (float)r);
-    if (mode == ORC_C32) return (double)mm_c32((float)l, (float)r);
+    if (mode == ORC_C32 || mode == ORC_C32K) return (double)mm_c32((float)l, (float)r);
     return mm_f64(l, r);
 }
 

@@ -27,6 +27,8 @@
  *               (TwoSum update).  Restates the product's FP32-compensated
  *               kernels so they can be checked bitwise; its accuracy is
  *               judged against ORC_F64.
+ *   ORC_C32K  : ORC_C32 with the face terms added to each cell by one Kahan
+ *               (compensated summation) step instead of the TwoSum chain.
  *
  * Parity pinning status (which check fixes which function) is listed per
  * function in shll_oracle.c.
@@ -40,7 +42,7 @@
 extern "C" {
 #endif
 
-enum { ORC_F64 = 0, ORC_REF32 = 1, ORC_C32 = 2 };
+enum { ORC_F64 = 0, ORC_REF32 = 1, ORC_C32 = 2, ORC_C32K = 3 };
 enum { ORC_BC_REFLECT = 0, ORC_BC_OUTFLOW = 1 };
 
 typedef struct {
@@ -80,8 +82,8 @@ void orc_constants(const orc_problem *pb, int mode, double out[6] /* gamma,R,cv,
 int orc_run(const orc_problem *pb, int mode, const double *prim_in, int64_t nsteps,
             double *prim_out, double *cons_out, int nthreads, double *elapsed_s);
 
-/* C32 only: also return the raw hi and lo float planes (nvar*nx*ny each). */
-int orc_run_c32_raw(const orc_problem *pb, const double *prim_in, int64_t nsteps,
+/* C32 / C32K only: also return the raw hi and lo float planes (nvar*nx*ny each). */
+int orc_run_c32_raw(const orc_problem *pb, int mode, const double *prim_in, int64_t nsteps,
                     float *hi_out, float *lo_out);
 
 /* Stage evaluation on whole arrays: the reference's Compute_U_from_P, Compute_P_from_U and

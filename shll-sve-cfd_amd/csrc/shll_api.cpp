@@ -120,7 +120,7 @@ static shll_status check_problem(const shll_problem* pb) {
     if (pb->dim != 1 && pb->dim != 2) return fail(SHLL_EINVAL, "dim must be 1 or 2 (got %d)", pb->dim);
     if (pb->order != 1 && pb->order != 2) return fail(SHLL_EINVAL, "order must be 1 or 2 (got %d)", pb->order);
     if (pb->bc != SHLL_BC_REFLECT && pb->bc != SHLL_BC_OUTFLOW) return fail(SHLL_EINVAL, "bad bc %d", pb->bc);
-    if (pb->precision < SHLL_F64 || pb->precision > SHLL_C32)
+    if (pb->precision < SHLL_F64 || pb->precision > SHLL_C32K)
         return fail(SHLL_EINVAL, "bad precision %d", pb->precision);
     if (pb->nx < 2 || pb->nx > (1 << 30)) return fail(SHLL_EINVAL, "nx out of range (%lld)", (long long)pb->nx);
     if (pb->dim == 1 && pb->ny != 1) return fail(SHLL_EINVAL, "ny must be 1 in 1D");
@@ -139,7 +139,7 @@ static shll_status ensure_device() {
 }
 
 static size_t word_bytes(int prec) { return prec == SHLL_F64 ? 8 : 4; }
-static int nwords(int prec) { return prec == SHLL_C32 ? 2 : 1; }
+static int nwords(int prec) { return prec == SHLL_C32 || prec == SHLL_C32K ? 2 : 1; }
 
 // ---------------------------------------------------------------------------
 // launch shapes and work lists

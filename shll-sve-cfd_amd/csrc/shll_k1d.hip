@@ -21,6 +21,7 @@ namespace shll {
 template <int PREC> struct Real { using T = double; };
 template <> struct Real<PREC_REF32> { using T = float; };
 template <> struct Real<PREC_C32> { using T = float; };
+template <> struct Real<PREC_C32K> { using T = float; };
 
 // Wall modes of a wavefront's segment (a wave-uniform choice, see k_step1d):
 //   WM_NONE   the segment is at least 3 cells from both ends: no boundary rule applies
@@ -141,7 +142,9 @@ __device__ __forceinline__ void step_ref(typename Real<PREC>::T (&u)[3][C], cons
 // ---------------------------------------------------------------------------
 // C32: float fluxes of the hi word, face terms t = lam*G added with TwoSum
 // ---------------------------------------------------------------------------
-template <int ORDER, int C, int WM>
+// KAHAN: the face terms t[m] - t[m+1] enter each cell by one Kahan step (lo holds
+// the part of the value not yet in hi) instead of the exact TwoSum chain
+template <int ORDER, int C, int WM, bool KAHAN>
 __device__ __forceinline__ void step_c32(float (&hi)[3][C], float (&lo)[3][C], const Inst& in, int gl) {
     using A = C32Arith;
     const float rcv = in.k.c_rcv, gR = in.k.c_gR, lam = in.k.c_lamx, hlam = 0.5f * lam;
@@ -177,12 +180,19 @@ __device__ __forceinline__ void step_c32(float (&hi)[3][C], float (&lo)[3][C], c
         }
 #pragma unroll
         for (int m = 0; m < C; ++m) {
-            float a = hi[v][m], b = lo[v][m], e;
-            two_sum(a, t[m], a, e);
-            b = b + e;
-            two_sum(a, -t[m + 1], a, e);
-            b = b + e;
-            fast_two_sum(a, b, hi[v][m], lo[v][m]);
+            if constexpr (KAHAN) {
+                const float y = (t[m] - t[m + 1]) + lo[v][m];
+                const float s = hi[v][m] + y;
+                lo[v][m] = y - (s - hi[v][m]);
+                hi[v][m] = s;
+            } else {
+                float a = hi[v][m], b = lo[v][m], e;
+                two_sum(a, t[m], a, e);
+                b = b + e;
+                two_sum(a, -t[m + 1], a, e);
+                b = b + e;
+                fast_two_sum(a, b, hi[v][m], lo[v][m]);
+            }
         }
     }
 }
@@ -192,10 +202,10 @@ __device__ __forceinline__ void step_c32(float (&hi)[3][C], float (&lo)[3][C], c
 // wall mode gets its own copy of loop and store, so the loop-carried state never
 // has to be moved between the copies' registers.
 // ---------------------------------------------------------------------------
-template <int ORDER, int C, int WM>
+template <int ORDER, int C, int WM, bool KAHAN>
 __device__ __forceinline__ void advance_c32(float (&hi)[3][C], float (&lo)[3][C], const Inst& in, const Item1& it,
                                             int gl, int ns, const StepArgs& a) {
-    for (int s = 0; s < ns; ++s) step_c32<ORDER, C, WM>(hi, lo, in, gl);
+    for (int s = 0; s < ns; ++s) step_c32<ORDER, C, WM, KAHAN>(hi, lo, in, gl);
     float* __restrict__ oh = (float*)a.out0 + in.off;
     float* __restrict__ ol = (float*)a.out1 + in.off;
     const int64_t P = a.plane;
@@ -252,7 +262,8 @@ __global__ __launch_bounds__(64 * WPB) void k_step1d(StepArgs a) {
     const int wm = !walls ? WM_NONE : (ORDER == 2 && in.bc == BC_REFLECT ? WM_RULES : WM_GHOST);
     constexpr int WR = ORDER == 2 ? WM_RULES : WM_GHOST;
 
-    if constexpr (PREC == PREC_C32) {
+    if constexpr (PREC == PREC_C32 || PREC == PREC_C32K) {
+        constexpr bool K = PREC == PREC_C32K;
         const float* __restrict__ ih = (const float*)a.in0 + in.off;
         const float* __restrict__ il = (const float*)a.in1 + in.off;
         float hi[3][C], lo[3][C];
@@ -266,9 +277,9 @@ __global__ __launch_bounds__(64 * WPB) void k_step1d(StepArgs a) {
                 lo[v][m] = ok ? il[v * P + g] : 0.0f;
             }
         }
-        if (wm == WM_NONE) advance_c32<ORDER, C, WM_NONE>(hi, lo, in, it, gl, ns, a);
-        else if (ORDER == 1 || wm == WM_GHOST) advance_c32<ORDER, C, WM_GHOST>(hi, lo, in, it, gl, ns, a);
-        else advance_c32<ORDER, C, WR>(hi, lo, in, it, gl, ns, a);
+        if (wm == WM_NONE) advance_c32<ORDER, C, WM_NONE, K>(hi, lo, in, it, gl, ns, a);
+        else if (ORDER == 1 || wm == WM_GHOST) advance_c32<ORDER, C, WM_GHOST, K>(hi, lo, in, it, gl, ns, a);
+        else advance_c32<ORDER, C, WR, K>(hi, lo, in, it, gl, ns, a);
     } else {
         const RT* __restrict__ iu = (const RT*)a.in0 + in.off;
         RT u[3][C];
@@ -317,6 +328,8 @@ hipError_t launch_step1d(int prec, int order, int C, int wpb, int n_items, hipSt
                                               : launch_o<PREC_REF32, 2>(C, wpb, nblocks, s, a);
     if (prec == PREC_C32) return order == 1 ? launch_o<PREC_C32, 1>(C, wpb, nblocks, s, a)
                                             : launch_o<PREC_C32, 2>(C, wpb, nblocks, s, a);
+    if (prec == PREC_C32K) return order == 1 ? launch_o<PREC_C32K, 1>(C, wpb, nblocks, s, a)
+                                             : launch_o<PREC_C32K, 2>(C, wpb, nblocks, s, a);
     return hipErrorInvalidValue;
 }
 

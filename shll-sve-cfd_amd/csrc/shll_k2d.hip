@@ -20,13 +20,14 @@ namespace shll {
 template <int PREC> struct Real2 { using T = double; };
 template <> struct Real2<PREC_REF32> { using T = float; };
 template <> struct Real2<PREC_C32> { using T = float; };
+template <> struct Real2<PREC_C32K> { using T = float; };
 
 // x and y split fluxes of one cell.  U: conserved (F64/REF32: the state; C32: hi).
 template <int PREC, class RT>
 __device__ __forceinline__ void fluxes2(const RT (&U)[4], const Inst& in, RT (&xp)[4], RT (&xm)[4], RT (&yp)[4],
                                         RT (&ym)[4]) {
     RT vx, vy, T, a, z1, z2, z3;
-    if constexpr (PREC == PREC_C32) {
+    if constexpr (PREC == PREC_C32 || PREC == PREC_C32K) {
         C32Arith::prim2(U[0], U[1], U[2], U[3], in.k.c_rcv, in.k.c_gR, vx, vy, T, a);
     } else {
         const RT cv = PREC == PREC_F64 ? (RT)in.k.d_cv : (RT)in.k.f_cv;
@@ -34,7 +35,7 @@ __device__ __forceinline__ void fluxes2(const RT (&U)[4], const Inst& in, RT (&x
         RefArith<RT>::prim2(U[0], U[1], U[2], U[3], cv, gR, vx, vy, T, a);
     }
     const RT P = U[0] * T;  // (rho*R)*T with R = 1 (base_shll_2d.c:24, :246)
-    if constexpr (PREC == PREC_C32) {
+    if constexpr (PREC == PREC_C32 || PREC == PREC_C32K) {
         // the compensated scheme's float form: fused multiply-adds (split_c32)
         C32Arith::zinv(div_fast(vx, a), a, z1, z2, z3);
         split_c32(U[1], U[0], z1, z2, z3, xp[0], xm[0]);
@@ -69,10 +70,10 @@ __device__ __forceinline__ int sgn_y(int v) { return v == 2 ? 1 : -1; }
 template <int PREC, int ORDER, int WPB>
 __global__ __launch_bounds__(64 * WPB) void k_step2d(StepArgs a) {
     using RT = typename Real2<PREC>::T;
-    using A = typename std::conditional<PREC == PREC_C32, C32Arith, RefArith<RT>>::type;
+    using A = typename std::conditional<PREC == PREC_C32 || PREC == PREC_C32K, C32Arith, RefArith<RT>>::type;
     constexpr int NW = 2 * ORDER + 1;  // rows of x fluxes kept: c-ORDER .. c+ORDER
     constexpr int NS = ORDER + 1;      // rows of state / y fluxes kept: c .. c+ORDER
-    constexpr bool CMP = PREC == PREC_C32;
+    constexpr bool CMP = PREC == PREC_C32 || PREC == PREC_C32K;  // (hi, lo) state
 
     const int wave = (int)blockIdx.x * WPB + uniform((int)threadIdx.x >> 6);
     const LaunchView lv = launch_view(a);
@@ -201,15 +202,23 @@ __global__ __launch_bounds__(64 * WPB) void k_step2d(StepArgs a) {
                 else
                     tt = j == ny - 1 ? lamy * (refl ? hpc + sy * hpc : hpc + hmc) : tt;
                 RT aa = su[0][v], bb = sl[0][v], e;
-                two_sum(aa, tl, aa, e);
-                bb = bb + e;
-                two_sum(aa, -tr, aa, e);
-                bb = bb + e;
-                two_sum(aa, tb, aa, e);
-                bb = bb + e;
-                two_sum(aa, -tt, aa, e);
-                bb = bb + e;
-                fast_two_sum(aa, bb, aa, e);
+                if constexpr (PREC == PREC_C32K) {
+                    // one Kahan step: lo (bb) is the part of the value not yet in hi
+                    const RT y = ((tl - tr) + (tb - tt)) + bb;
+                    const RT s2 = aa + y;
+                    e = y - (s2 - aa);
+                    aa = s2;
+                } else {
+                    two_sum(aa, tl, aa, e);
+                    bb = bb + e;
+                    two_sum(aa, -tr, aa, e);
+                    bb = bb + e;
+                    two_sum(aa, tb, aa, e);
+                    bb = bb + e;
+                    two_sum(aa, -tt, aa, e);
+                    bb = bb + e;
+                    fast_two_sum(aa, bb, aa, e);
+                }
                 if (jown && c < it.ix0 + it.iw) {
                     o0p[v * P + o] = aa;
                     o1p[v * P + o] = e;
@@ -235,6 +244,8 @@ hipError_t launch_step2d(int prec, int order, int wpb, int n_items, hipStream_t 
                                               : launch2_o<PREC_REF32, 2>(wpb, nblocks, s, a);
     if (prec == PREC_C32) return order == 1 ? launch2_o<PREC_C32, 1>(wpb, nblocks, s, a)
                                             : launch2_o<PREC_C32, 2>(wpb, nblocks, s, a);
+    if (prec == PREC_C32K) return order == 1 ? launch2_o<PREC_C32K, 1>(wpb, nblocks, s, a)
+                                             : launch2_o<PREC_C32K, 2>(wpb, nblocks, s, a);
     return hipErrorInvalidValue;
 }
 

@@ -6,7 +6,7 @@
 
 namespace shll {
 
-enum Prec : int { PREC_F64 = 0, PREC_REF32 = 1, PREC_C32 = 2 };
+enum Prec : int { PREC_F64 = 0, PREC_REF32 = 1, PREC_C32 = 2, PREC_C32K = 3 };
 enum Bc : int { BC_REFLECT = 0, BC_OUTFLOW = 1 };
 
 // Per-instance scheme constants, computed on the host (never folded on the device):

@@ -11,7 +11,7 @@
  * The reference's compile-time constants become flags.  Defaults reproduce
  * base_shll.c (1D, N=256, t=0.2, Sod, reflective walls, reference float arithmetic).
  *
- *   shll_run [--dim 1|2] [--order 1|2] [--nx N] [--ny N] [--precision f64|ref32|c32]
+ *   shll_run [--dim 1|2] [--order 1|2] [--nx N] [--ny N] [--precision f64|ref32|c32|c32k]
  *            [--bc reflect|outflow] [--ic sod|implosion|four-shocks|circle]
  *            [--t-end T] [--steps K] [--out results.dat] [--sums]
  */
@@ -24,7 +24,7 @@
 static void usage(void)
 {
     fprintf(stderr,
-            "usage: shll_run [--dim 1|2] [--order 1|2] [--nx N] [--ny N] [--precision f64|ref32|c32]\n"
+            "usage: shll_run [--dim 1|2] [--order 1|2] [--nx N] [--ny N] [--precision f64|ref32|c32|c32k]\n"
             "                [--bc reflect|outflow] [--ic sod|implosion|four-shocks|circle]\n"
             "                [--t-end T] [--steps K] [--out PATH] [--sums]\n");
 }
@@ -53,7 +53,10 @@ int main(int argc, char **argv)
         else if (!strcmp(a, "--nx")) pb.nx = atoll(v);
         else if (!strcmp(a, "--ny")) pb.ny = atoll(v);
         else if (!strcmp(a, "--precision"))
-            pb.precision = !strcmp(v, "f64") ? SHLL_F64 : !strcmp(v, "c32") ? SHLL_C32 : SHLL_REF32;
+            pb.precision = !strcmp(v, "f64")    ? SHLL_F64
+                           : !strcmp(v, "c32")  ? SHLL_C32
+                           : !strcmp(v, "c32k") ? SHLL_C32K
+                                                : SHLL_REF32;
         else if (!strcmp(a, "--bc")) { pb.bc = !strcmp(v, "outflow") ? SHLL_BC_OUTFLOW : SHLL_BC_REFLECT; bc_set = 1; }
         else if (!strcmp(a, "--ic")) ic = v;
         else if (!strcmp(a, "--t-end")) t_end = (float)atof(v);

@@ -21,9 +21,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.dirname(HERE)
 LIB_PATH = os.environ.get("SHLL_AMD_LIB", os.path.join(PKG, "lib", "libshll_amd.so"))
 
-F64, REF32, C32 = 0, 1, 2
+F64, REF32, C32, C32K = 0, 1, 2, 3
 BC_REFLECT, BC_OUTFLOW = 0, 1
-PREC_NAMES = {"f64": F64, "ref32": REF32, "c32": C32}
+PREC_NAMES = {"f64": F64, "ref32": REF32, "c32": C32, "c32k": C32K}
 
 OK, EINVAL, ENOMEM, EDEVICE, ENONFINITE, ESTATE = 0, 1, 2, 3, 4, 5
 
@@ -217,7 +217,7 @@ class Solver:
         return out
 
     def state_raw(self):
-        """F64: (nv, N) float64; REF32: (nv, N) float32; C32: (hi, lo) float32 pair."""
+        """F64: (nv, N) float64; REF32: (nv, N) float32; C32 / C32K: (hi, lo) float32 pair."""
         nb = lib().shll_state_bytes(self.h)
         buf = np.empty(nb, dtype=np.uint8)
         _check(lib().shll_get_state_raw(self.h, buf.ctypes.data_as(ctypes.c_void_p), nb))

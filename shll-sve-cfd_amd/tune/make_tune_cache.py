@@ -24,15 +24,17 @@ def main():
     # batched FP32-compensated second-order tubes (the benchmark batch); the trial
     # window (2048 steps, every tube of >= 1280 cells still running) weights the tube
     # sizes roughly as the whole run does
-    pbs, prims, steps = S.tube_batch(20261202, 4096, precision=S.C32)
-    b = S.Batch(pbs, prims, steps)
-    b.autotune(2048)
-    print("batch c32 o2:", b.get_launch(), flush=True)
-    b.close()
+    for prec in (S.C32, S.C32K):
+        pbs, prims, steps = S.tube_batch(20261202, 4096, precision=prec)
+        b = S.Batch(pbs, prims, steps)
+        b.autotune(2048)
+        print("batch prec%d o2:" % prec, b.get_launch(), flush=True)
+        b.close()
     # single instances: 1D first order 1024 cells, 1D second order 65536 cells
     for pb, trial in ((S.problem(1, 1, 1024, precision=S.F64), 400),
                       (S.problem(1, 2, 65536, precision=S.F64), 2000),
-                      (S.problem(1, 2, 65536, precision=S.C32), 2000)):
+                      (S.problem(1, 2, 65536, precision=S.C32), 2000),
+                      (S.problem(1, 2, 65536, precision=S.C32K), 2000)):
         with S.Solver(pb, S.ic_tube(pb.nx)) as s:
             s.autotune(trial)
             print("single dim1 order%d prec%d N=%d:" % (pb.order, pb.precision, pb.nx), s.get_launch(), flush=True)

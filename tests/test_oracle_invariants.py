@@ -20,7 +20,7 @@ def textbook_flux_1d(prim):
     return np.stack([rho * u, rho * u * u + p, u * (E + p)]), np.stack([rho, rho * u, E]), np.sqrt(GAMMA * R * T)
 
 
-@pytest.mark.parametrize("mode", [0, 1, 2])
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
 @pytest.mark.parametrize("dim,order,bc", [(1, 1, 0), (1, 1, 1), (1, 2, 1), (1, 2, 0), (2, 1, 0), (2, 2, 1)])
 def test_rest_state_is_a_fixed_point(orc, mode, dim, order, bc):
     """A uniform gas at rest must stay exactly at rest with any wall treatment."""
@@ -56,7 +56,7 @@ def test_split_flux_identity_and_rest_closed_form(orc):
     assert np.max(np.abs(st1["fm"]) / np.abs(F1).max(axis=0)) < 1e-13
 
 
-@pytest.mark.parametrize("mode", [0, 2])
+@pytest.mark.parametrize("mode", [0, 2, 3])
 @pytest.mark.parametrize("order,bc", [(1, 0), (2, 1)])
 def test_mirror_symmetry(orc, mode, order, bc):
     """Reflecting the data about x = 1/2 reflects the solution (rho even, u odd)."""
@@ -72,9 +72,10 @@ def test_mirror_symmetry(orc, mode, order, bc):
     np.testing.assert_allclose(-pm[1][::-1], p[1], atol=tol * np.abs(p[1]).max())
 
 
-@pytest.mark.parametrize("mode,tol", [(0, 1e-14), (2, 1e-13)])
+@pytest.mark.parametrize("mode,tol", [(0, 1e-14), (2, 1e-13), (3, 1e-10)])
 def test_conservation_with_reflective_walls(orc, mode, tol):
-    """Mass and energy are conserved; F64 to round-off, C32 (telescoping face terms) too."""
+    """Mass and energy are conserved; F64 to round-off, C32 (telescoping face terms) too;
+    C32K's Kahan update to ~1e-12 (the net face term of a cell is rounded once)."""
     pb = orc.problem(1, 1, 2048)
     ic = orc.ic_tube(2048)
     c0 = orc.stage(pb, mode, ic)["cons"].sum(axis=1)
@@ -230,10 +231,22 @@ def test_minmod_brute_force(orc):
     assert orc.minmod(orc.F64, 1e-30, -1e-30) == 0.0
 
 
-def test_c32_first_order_accuracy(orc):
-    """The compensated scheme stays within L1(rho) <= 1e-6 of FP64 on first-order Sod."""
+@pytest.mark.parametrize("mode", [2, 3])
+def test_c32_first_order_accuracy(orc, mode):
+    """The compensated schemes stay within L1(rho) <= 1e-6 of FP64 on first-order Sod."""
     pb = orc.problem(1, 1, 1024)
     ic = orc.ic_tube(1024)
     pf, _ = orc.run(pb, orc.F64, ic, 1639)
-    pc, _ = orc.run(pb, orc.C32, ic, 1639)
+    pc, _ = orc.run(pb, mode, ic, 1639)
     assert np.abs(pc[0] - pf[0]).sum() / 1024 <= 1e-6
+
+
+def test_kahan_and_twosum_updates_agree(orc):
+    """The two compensated updates differ only in how the same face terms are added:
+    their states stay within a few FP32 ulps of each other (first order, 400 steps)."""
+    pb = orc.problem(1, 1, 512)
+    ic = icgen.random_1d(512, 21, block=16)
+    _, c2 = orc.run(pb, orc.C32, ic, 400)
+    _, c3 = orc.run(pb, orc.C32K, ic, 400)
+    assert np.abs(c3 - c2).max() <= 1e-6 * np.abs(c2).max()
+    assert not np.array_equal(c2, c3)  # they are different schemes
