@@ -15,6 +15,10 @@
 // (base_shll_2d.c:150-216) or outflow (2nd_order_base_shll.c:214-323).
 #include "shll_device.h"
 
+#ifndef PF_DEPTH
+#define PF_DEPTH 2
+#endif
+
 namespace shll {
 
 template <int PREC> struct Real2 { using T = double; };
@@ -111,7 +115,24 @@ __global__ __launch_bounds__(64 * WPB) void k_step2d(StepArgs a) {
     const RT lamx = PREC == PREC_F64 ? (RT)in.k.d_lamx : (PREC == PREC_REF32 ? (RT)in.k.f_lamx : (RT)in.k.c_lamx);
     const RT lamy = PREC == PREC_F64 ? (RT)in.k.d_lamy : (PREC == PREC_REF32 ? (RT)in.k.f_lamy : (RT)in.k.c_lamy);
 
+    // The state of row t+D is loaded while row t is processed (before row t's stores:
+    // one counter tracks loads and stores, so a row's wait never covers the stores of
+    // the rows before it); D rows of loads in flight per wave keep HBM busy.
+    constexpr int D = PF_DEPTH;
+    RT nu[D][4], nl[D][4];
+    auto load_row = [&](int t, RT (&U)[4], RT (&Ul)[4]) {
+        if (t >= 0 && t < nx) {
+            const int64_t o = (int64_t)t * ny + jc;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                U[v] = jok ? i0p[v * P + o] : (RT)(v == 0 ? 1.0 : (v == 3 ? 2.5 : 0.0));
+                Ul[v] = CMP && jok ? i1p[v * P + o] : (RT)0;
+            }
+        }
+    };
     const int tend = it.ix0 + it.iw + ORDER;
+#pragma unroll
+    for (int d = 0; d < D; ++d) load_row(it.ix0 - ORDER + d, nu[d], nl[d]);
     for (int t = it.ix0 - ORDER; t < tend; ++t) {
         // rotate the windows by one row
 #pragma unroll
@@ -125,17 +146,21 @@ __global__ __launch_bounds__(64 * WPB) void k_step2d(StepArgs a) {
                 yp[w][v] = yp[w + 1][v]; ym[w][v] = ym[w + 1][v];
                 su[w][v] = su[w + 1][v]; sl[w][v] = sl[w + 1][v];
             }
-        if (t >= 0 && t < nx) {
-            const int64_t o = (int64_t)t * ny + jc;
-            RT U[4];
+        const bool tin = t >= 0 && t < nx;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            su[NS - 1][v] = nu[0][v];
+            sl[NS - 1][v] = nl[0][v];
+        }
+#pragma unroll
+        for (int d = 0; d + 1 < D; ++d)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                U[v] = jok ? i0p[v * P + o] : (RT)(v == 0 ? 1.0 : (v == 3 ? 2.5 : 0.0));
-                su[NS - 1][v] = U[v];
-                if constexpr (CMP) sl[NS - 1][v] = jok ? i1p[v * P + o] : (RT)0;
+                nu[d][v] = nu[d + 1][v];
+                nl[d][v] = nl[d + 1][v];
             }
-            fluxes2<PREC, RT>(U, in, xp[NW - 1], xm[NW - 1], yp[NS - 1], ym[NS - 1]);
-        }
+        if (t + D < tend) load_row(t + D, nu[D - 1], nl[D - 1]);
+        if (tin) fluxes2<PREC, RT>(su[NS - 1], in, xp[NW - 1], xm[NW - 1], yp[NS - 1], ym[NS - 1]);
         const int c = t - ORDER;  // row to update
         if (c < it.ix0 || c >= nx) continue;
         const int64_t o = (int64_t)c * ny + jc;
