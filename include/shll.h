@@ -111,10 +111,18 @@ shll_status shll_get_conserved(shll_ctx *ctx, double *out_soa);
  * C32 nvar*N hi floats then nvar*N lo floats.  bytes must equal shll_state_bytes(). */
 size_t shll_state_bytes(const shll_ctx *ctx);
 shll_status shll_get_state_raw(shll_ctx *ctx, void *out, size_t bytes);
+/* Restart: replace the state by a buffer of shll_get_state_raw's layout (a checkpoint of a
+ * context with the same problem).  Continuing from it gives bit for bit the results of the
+ * uninterrupted run.  No reference counterpart (the reference has no checkpoint). */
+shll_status shll_set_state_raw(shll_ctx *ctx, const void *in, size_t bytes);
 /* Replaces Save_Results (base_shll.c:180-192, base_shll_2d.c:322-340): the same text
  * format, "%e\t%e\t%e\t%e\n" (x, rho, u, T) in 1D, six columns (x, y, rho, u, v, T) in 2D,
  * written from float values as the reference does. */
 shll_status shll_write_results(shll_ctx *ctx, const char *path);
+/* Replaces the ten-column Save_Results of omp-vectors/2nd_order_arm_sve.c:921-940: in 2D
+ * x, y, rho, u, v, T, u0, u1, u2, u3 (primitives then conserved variables); in 1D the same
+ * layout with seven columns (x, rho, u, T, u0, u1, u2).  Float values, "%e" tab-separated. */
+shll_status shll_write_results_full(shll_ctx *ctx, const char *path);
 
 /* FP64 sums of the conserved variables times the cell volume (mass, momentum(s),
  * energy); out has nvar entries.  No reference counterpart (diagnostic). */

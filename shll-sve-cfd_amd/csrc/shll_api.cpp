@@ -807,6 +807,24 @@ extern "C" shll_status shll_get_state_raw(shll_ctx* ctx, void* out, size_t bytes
     return SHLL_OK;
 }
 
+extern "C" shll_status shll_set_state_raw(shll_ctx* ctx, const void* in, size_t bytes) {
+    if (!ctx || !in) return fail(SHLL_EINVAL, "NULL argument");
+    if (bytes != shll_state_bytes(ctx)) return fail(SHLL_EINVAL, "bytes must be %zu", shll_state_bytes(ctx));
+    Engine* E = ctx->E;
+    HIPCHECK(hipSetDevice(E->device));
+    const size_t wb = word_bytes(E->prec);
+    const int64_t nc = E->inst_cells[0];
+    const char* p = (const char*)in;
+    for (int w = 0; w < nwords(E->prec); ++w)
+        for (int v = 0; v < E->nv; ++v) {
+            HIPCHECK(hipMemcpyAsync((char*)E->buf[E->cur][w] + (size_t)v * E->plane * wb, p, nc * wb,
+                                    hipMemcpyHostToDevice, E->stream));
+            p += nc * wb;
+        }
+    HIPCHECK(hipStreamSynchronize(E->stream));
+    return SHLL_OK;
+}
+
 // Save_Results (base_shll.c:180-192; base_shll_2d.c:322-340): values pass through float
 // as in the reference's arrays; cx = (i+0.5)*DX evaluated in double and stored to float.
 extern "C" shll_status shll_write_results(shll_ctx* ctx, const char* path) {
@@ -837,6 +855,35 @@ extern "C" shll_status shll_write_results(shll_ctx* ctx, const char* path) {
                              (double)(float)p[nc + c], (double)(float)p[2 * nc + c], (double)(float)p[3 * nc + c]);
             }
     }
+    if (std::fclose(f) != 0) return fail(SHLL_EINVAL, "write to %s failed", path);
+    return SHLL_OK;
+}
+
+// The omp-vectors Save_Results (2nd_order_arm_sve.c:921-940): primitives and conserved
+// variables of every cell, as floats, cell centres (i+0.5)*DX computed in double.
+extern "C" shll_status shll_write_results_full(shll_ctx* ctx, const char* path) {
+    if (!ctx || !path) return fail(SHLL_EINVAL, "NULL argument");
+    const shll_problem& pb = ctx->pb;
+    const int64_t nc = pb.nx * pb.ny;
+    const int nv = pb.dim == 1 ? 3 : 4;
+    std::vector<double> p((size_t)nc * nv), u((size_t)nc * nv);
+    shll_status st = shll_get_primitives(ctx, p.data());
+    if (st == SHLL_OK) st = shll_get_conserved(ctx, u.data());
+    if (st != SHLL_OK) return st;
+    FILE* f = std::fopen(path, "w");
+    if (!f) return fail(SHLL_EINVAL, "cannot open %s", path);
+    const float DX = (float)pb.lx / (float)pb.nx;
+    const float DY = pb.dim == 2 ? (float)pb.ly / (float)pb.ny : 1.0f;
+    for (int64_t i = 0; i < pb.nx; ++i)
+        for (int64_t j = 0; j < pb.ny; ++j) {
+            const int64_t c = i * pb.ny + j;
+            const float cx = (float)(((double)i + 0.5) * (double)DX);
+            std::fprintf(f, "%e", (double)cx);
+            if (pb.dim == 2) std::fprintf(f, "\t%e", (double)(float)(((double)j + 0.5) * (double)DY));
+            for (int v = 0; v < nv; ++v) std::fprintf(f, "\t%e", (double)(float)p[(size_t)v * nc + c]);
+            for (int v = 0; v < nv; ++v) std::fprintf(f, "\t%e", (double)(float)u[(size_t)v * nc + c]);
+            std::fputc('\n', f);
+        }
     if (std::fclose(f) != 0) return fail(SHLL_EINVAL, "write to %s failed", path);
     return SHLL_OK;
 }

@@ -63,7 +63,8 @@ class DomainSpec(ctypes.Structure):
 
 EXPORTS = [
     "shll_init", "shll_step", "shll_run_to_time", "shll_steps_to_time", "shll_get_primitives",
-    "shll_get_conserved", "shll_state_bytes", "shll_get_state_raw", "shll_write_results",
+    "shll_get_conserved", "shll_state_bytes", "shll_get_state_raw", "shll_set_state_raw", "shll_write_results",
+    "shll_write_results_full",
     "shll_conserved_sums", "shll_check_state", "shll_compare", "shll_compare_host", "shll_free",
     "shll_set_launch", "shll_get_launch", "shll_autotune", "shll_set_timing", "shll_last_timing",
     "shll_batch_init", "shll_batch_advance", "shll_batch_run", "shll_batch_get_primitives",
@@ -98,7 +99,9 @@ def lib():
         "shll_get_conserved": (c, [vp, dp]),
         "shll_state_bytes": (ctypes.c_size_t, [vp]),
         "shll_get_state_raw": (c, [vp, vp, ctypes.c_size_t]),
+        "shll_set_state_raw": (c, [vp, vp, ctypes.c_size_t]),
         "shll_write_results": (c, [vp, ctypes.c_char_p]),
+        "shll_write_results_full": (c, [vp, ctypes.c_char_p]),
         "shll_conserved_sums": (c, [vp, dp]),
         "shll_check_state": (c, [vp, ctypes.POINTER(i64)]),
         "shll_compare": (c, [vp, vp, ctypes.POINTER(Errors)]),
@@ -229,8 +232,20 @@ class Solver:
         f = f.reshape(2, self.nv, self.ncell)
         return f[0], f[1]
 
+    def set_state_raw(self, raw):
+        """Restart from a checkpoint: raw as returned by state_raw() (F64 / REF32 array, or
+        the (hi, lo) pair of the compensated forms)."""
+        if isinstance(raw, tuple):
+            raw = np.concatenate([np.ascontiguousarray(r).ravel() for r in raw])
+        buf = np.ascontiguousarray(raw).view(np.uint8).ravel()
+        _check(lib().shll_set_state_raw(self.h, buf.ctypes.data_as(ctypes.c_void_p), buf.size))
+
     def write_results(self, path):
         _check(lib().shll_write_results(self.h, str(path).encode()))
+
+    def write_results_full(self, path):
+        """Primitives and conserved variables (the ten-column 2D format of omp-vectors)."""
+        _check(lib().shll_write_results_full(self.h, str(path).encode()))
 
     def conserved_sums(self):
         out = np.zeros(4)
@@ -427,3 +442,20 @@ def domain_batch(seed, count, smin=256, smax=2048, precision=C32, order=1):
         pbs.append(problem(2, order, d.nx, d.ny, precision=precision, lx=d.lx, ly=d.ly))
         prims.append(ic_domain(d))
     return pbs, prims
+
+
+def read_results(path, nx, ny=1):
+    """Read a results.dat written by write_results / write_results_full (or by the
+    reference's Save_Results) into named (nx, ny) arrays -- the reading half of the
+    reference's plotting script (base-omp/fetch_local.py:13-27), without S3 or a plot."""
+    data = np.loadtxt(path, ndmin=2)
+    dim = 1 if ny == 1 else 2
+    names = {(1, 4): ["x", "rho", "u", "T"],
+             (1, 7): ["x", "rho", "u", "T", "u0", "u1", "u2"],
+             (2, 6): ["x", "y", "rho", "u", "v", "T"],
+             (2, 10): ["x", "y", "rho", "u", "v", "T", "u0", "u1", "u2", "u3"]}.get((dim, data.shape[1]))
+    if names is None or data.shape[0] != nx * ny:
+        raise ValueError("%s: %d rows x %d columns does not match a %dD results file of %d cells"
+                         % (path, data.shape[0], data.shape[1], dim, nx * ny))
+    shape = (nx,) if dim == 1 else (nx, ny)
+    return {n: data[:, k].reshape(shape) for k, n in enumerate(names)}

@@ -82,3 +82,20 @@ def test_no_cpu_fallback_without_a_gpu():
     with pytest.raises(S.ShllError) as e:
         S.Solver(pb, S.ic_tube(64))
     assert e.value.code == S.EDEVICE
+
+
+def test_results_reader_reads_the_reference_results_file(tmp_path):
+    """read_results parses the reference's own results.dat (base_shll.c, N = 256, 410 steps)
+    into the arrays the golden run holds."""
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "o1_1d_main.npz"), allow_pickle=False)
+    path = tmp_path / "results.dat"
+    path.write_bytes(bytes(g["results_dat"]))
+    r = S.read_results(path, 256)
+    assert sorted(r) == ["T", "rho", "u", "x"]
+    # "%e" keeps 7 significant digits
+    np.testing.assert_allclose(r["rho"], g["p_410"][0], rtol=6e-7)
+    np.testing.assert_allclose(r["u"], g["p_410"][1], rtol=6e-7, atol=1e-12)
+    np.testing.assert_allclose(r["T"], g["p_410"][2], rtol=6e-7)
+    np.testing.assert_allclose(r["x"], (np.arange(256) + 0.5) / 256, rtol=1e-6)
+    with pytest.raises(ValueError):
+        S.read_results(path, 255)
