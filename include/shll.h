@@ -5,6 +5,8 @@
  *   base-c/base_shll.c          (1D, first order, reflective walls)
  *   base-c/base_shll_2d.c       (2D, first order, reflective walls)
  *   base-c/2nd_order_base_shll.c(2D, second order, outflow)
+ *   base-omp/2nd_order_base_shll.c (the same with the MC limiter; its OpenMP worksharing
+ *                                    is replaced by the GPU launch)
  * whose only caller is main() (base_shll.c:198-225).  The reference routines are
  * void functions over file-scope float arrays; here the state lives on the GPU
  * behind an opaque context, and each call maps to the routines it replaces (cited
@@ -55,9 +57,17 @@ typedef enum {
     SHLL_BC_OUTFLOW = 1   /* 2nd_order_base_shll.c:214-256,280-323 */
 } shll_bc;
 
+/* Slope limiter of the second-order scheme. */
+typedef enum {
+    SHLL_LIM_MINMOD = 0,  /* minmod of the one-sided differences (2nd_order_base_shll.c:268-276,336-343) */
+    SHLL_LIM_MC = 1       /* minmod(0.5*(f[i+1]-f[i-1]), theta*minmod(one-sided differences)),
+                             base-omp/2nd_order_base_shll.c:44,317-325,385-392 (theta = alpha = 1.25) */
+} shll_limiter;
+
 /* Problem description: the reference's compile-time constants made runtime
- * (base_shll.c:16-26, base_shll_2d.c:21-35).  gamma = 1.4, R = 1, CFL = 0.25 as in
- * the reference; DT = (CFL/(R+1))*DX. */
+ * (base_shll.c:16-26, base_shll_2d.c:21-35, base-omp/2nd_order_base_shll.c:29-44).
+ * gamma = 1.4, R = 1, CFL = 0.25 as in the reference; DT = (CFL/(R+1))*DX.  A
+ * zero-initialised tail (limiter, reserved, theta) selects the minmod limiter. */
 typedef struct {
     int32_t dim;        /* 1 or 2 */
     int32_t order;      /* 1 (base_shll*.c) or 2 (2nd_order_base_shll.c) */
@@ -67,6 +77,9 @@ typedef struct {
     int64_t ny;         /* cells in y (1 in 1D, >= 2 in 2D) */
     double lx;          /* domain length in x (reference L = 1) */
     double ly;          /* domain length in y (reference H = 1) */
+    int32_t limiter;    /* shll_limiter (order 2 only; SHLL_LIM_MINMOD for order 1) */
+    int32_t reserved;   /* 0 */
+    double theta;       /* SHLL_LIM_MC weight in [1, 2]; the reference's is 1.25 */
 } shll_problem;
 
 typedef struct shll_ctx shll_ctx;
@@ -198,6 +211,8 @@ void shll_ic_tube(int64_t n, double x0, double rhoL, double uL, double TL, doubl
 void shll_ic_implosion(int64_t nx, int64_t ny, double *prim_soa);
 /* 2nd_order_base_shll.c:133-148 four shocks. */
 void shll_ic_four_shocks(int64_t nx, int64_t ny, double *prim_soa);
+/* base-omp/2nd_order_base_shll.c:149-167 four contacts ("Configuration 6"). */
+void shll_ic_four_contacts(int64_t nx, int64_t ny, double *prim_soa);
 /* Circle of (rho_in, T_in) for cell centres strictly inside radius r about (cx, cy). */
 void shll_ic_circle(int64_t nx, int64_t ny, double lx, double ly, double cx, double cy, double r, double rho_in,
                     double T_in, double rho_out, double T_out, double *prim_soa);

@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, "liboracle.so")
 
 F64, REF32, C32, C32K = 0, 1, 2, 3
 BC_REFLECT, BC_OUTFLOW = 0, 1
+LIM_MINMOD, LIM_MC = 0, 1
 
 
 class Problem(ctypes.Structure):
@@ -23,17 +24,19 @@ class Problem(ctypes.Structure):
         ("bc", ctypes.c_int32),
         ("nx", ctypes.c_int32),
         ("ny", ctypes.c_int32),
-        ("pad", ctypes.c_int32),
+        ("limiter", ctypes.c_int32),
         ("lx", ctypes.c_double),
         ("ly", ctypes.c_double),
+        ("theta", ctypes.c_double),
     ]
 
 
-def problem(dim, order, nx, ny=1, bc=None, lx=1.0, ly=1.0):
-    """The reference pairs first order with reflective walls and second order with outflow."""
+def problem(dim, order, nx, ny=1, bc=None, lx=1.0, ly=1.0, limiter=LIM_MINMOD, theta=1.25):
+    """The reference pairs first order with reflective walls and second order with outflow.
+    limiter LIM_MC with theta = 1.25 is base-omp/2nd_order_base_shll.c:44,317-325."""
     if bc is None:
         bc = BC_REFLECT if order == 1 else BC_OUTFLOW
-    return Problem(dim, order, bc, nx, ny if dim == 2 else 1, 0, lx, ly)
+    return Problem(dim, order, bc, nx, ny if dim == 2 else 1, limiter, lx, ly, theta)
 
 
 _lib = None
@@ -60,6 +63,9 @@ def lib():
         L.orc_ic_tube.argtypes = [ctypes.c_int] + [ctypes.c_double] * 7 + [dp]
         L.orc_ic_implosion.argtypes = [ctypes.c_int, ctypes.c_int, dp]
         L.orc_ic_four_shocks.argtypes = [ctypes.c_int, ctypes.c_int, dp]
+        L.orc_ic_four_contacts.argtypes = [ctypes.c_int, ctypes.c_int, dp]
+        L.orc_slope.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_double] * 4
+        L.orc_slope.restype = ctypes.c_double
         L.orc_ic_circle.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_double] * 9 + [dp]
         _lib = L
     return _lib
@@ -132,6 +138,16 @@ def stage(pb, mode, prim):
 
 def minmod(mode, l, r):
     return lib().orc_minmod(mode, float(l), float(r))
+
+
+def slope(mode, limiter, theta, ql, qc, qr):
+    return lib().orc_slope(mode, limiter, float(theta), float(ql), float(qc), float(qr))
+
+
+def ic_four_contacts(nx, ny):
+    out = np.zeros((4, nx * ny))
+    lib().orc_ic_four_contacts(nx, ny, _dp(out))
+    return out
 
 
 def ic_tube(n, x0=0.5, rhoL=10.0, uL=0.0, TL=1.0, rhoR=1.0, uR=0.0, TR=1.0):

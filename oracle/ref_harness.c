@@ -5,7 +5,8 @@
  * reference_main); the binaries land in oracle/_ref/ and are never committed.
  *
  * The reference keeps its state in file-scope float arrays with external linkage
- * (base_shll.c:9-15, base_shll_2d.c:9-20, 2nd_order_base_shll.c:9-28) and exposes
+ * (base_shll.c:9-15, base_shll_2d.c:9-20, 2nd_order_base_shll.c:9-28, and the same in
+ * base-omp/2nd_order_base_shll.c:9-29) and exposes
  * its stages as external void functions, so this file can
  *   - run the reference's own main() and dump the arrays when Free_Memory() makes
  *     its first free() call (link with -Wl,--wrap=free), or
@@ -35,7 +36,13 @@ extern float *p0, *p1, *p2, *u0, *u1, *u2, *a;
 extern float *p0, *p1, *p2, *p3, *u0, *u1, *u2, *u3, *a;
 #endif
 extern const int N;
+#if defined(REF_OMP_O2_2D)
+/* base-omp/2nd_order_base_shll.c counts steps in a local of main (:626); the "main"
+ * mode's dump then reports 0 steps */
+int NO_STEPS = 0;
+#else
 extern int NO_STEPS;
+#endif
 
 void Allocate_and_Init_Memory(void);
 void Compute_U_from_P(void);

@@ -14,6 +14,10 @@
  *       shocks (256^2, 1/10 steps), plus injected random / circle / y-uniform Sod
  *       initial data.  The derived 1D second-order scheme is pinned by the 2D
  *       second-order reference run on y-uniform Sod data (every row equal).
+ *   REF32 MC limiter: bitwise against base-omp/2nd_order_base_shll.c at its 1024^2
+ *       size (four contacts, random, y-uniform data; built with clang, see Makefile).
+ *   MC slope, every mode: equals the textbook minmod(theta*dl, (dl+dr)/2, theta*dr)
+ *       on dyadic data, minmod at theta = 1; MC beats minmod in L1 on Sod.
  *   F64: same source text as REF32 (double instantiation); additionally pinned by
  *       invariants: rest state is a fixed point, F+ + F- = F, M=0 closed form,
  *       mirror symmetry of a mirrored Sod run, mass/energy conservation with
@@ -62,6 +66,8 @@ static int orc_check(const orc_problem *pb)
     if (pb->dim != 1 && pb->dim != 2) return -1;
     if (pb->order != 1 && pb->order != 2) return -1;
     if (pb->bc != ORC_BC_REFLECT && pb->bc != ORC_BC_OUTFLOW) return -1;
+    if (pb->limiter != ORC_LIM_MINMOD && pb->limiter != ORC_LIM_MC) return -1;
+    if (pb->limiter == ORC_LIM_MC && (pb->order != 2 || !(pb->theta >= 1.0 && pb->theta <= 2.0))) return -1;
     if (pb->nx < 2) return -1;
     if (pb->dim == 1 && pb->ny != 1) return -1;
     if (pb->dim == 2 && pb->ny < 2) return -1;
@@ -96,8 +102,8 @@ int64_t orc_steps_to_time(const orc_problem *pb, float t_end)
 /* conserves them.                                                            */
 /* ------------------------------------------------------------------------- */
 typedef struct {
-    int nx, ny, n, nv, dim, order, bc, kahan;
-    float R, gR, rcv, lamx, lamy, hlamx, hlamy;
+    int nx, ny, n, nv, dim, order, bc, kahan, lim;
+    float R, gR, rcv, lamx, lamy, hlamx, hlamy, theta;
     double cv64, gR64;
     float *hi[4], *lo[4], *fp[4], *fm[4], *hp[4], *hm[4];
     void *block;
@@ -131,6 +137,8 @@ static int c32_alloc(c32grid *g, const orc_problem *pb)
     memset(g, 0, sizeof(*g));
     g->nx = pb->nx; g->ny = pb->ny; g->n = (int)n; g->nv = orc_nvar(pb);
     g->dim = pb->dim; g->order = pb->order; g->bc = pb->bc;
+    g->lim = pb->limiter;
+    g->theta = (float)pb->theta;
     make_cst_f64(&k, pb->nx, pb->ny, pb->lx, pb->dim == 2 ? pb->ly : 1.0);
     g->R = (float)k.R;
     g->gR = (float)k.gR;
@@ -259,11 +267,16 @@ static const float sx1_c32[3] = {-1.0f, 1.0f, -1.0f};
 static const float sx2_c32[4] = {-1.0f, 1.0f, -1.0f, -1.0f};
 static const float sy2_c32[4] = {-1.0f, -1.0f, 1.0f, -1.0f};
 
-/* slope of the split flux q at line position s (0 on the two boundary cells) */
-static float c32_slope(const float *q, int c, int s, int stride, int len)
+/* slope of the split flux q at line position s (0 on the two boundary cells):
+ * minmod of the one-sided differences, or with the MC limiter
+ * mm(0.5*(q_r - q_l), theta*mm(q_c - q_l, q_r - q_c)), all in float */
+static float c32_slope(const c32grid *g, const float *q, int c, int s, int stride, int len)
 {
+    float m;
     if (s <= 0 || s >= len - 1) return 0.0f;
-    return mm_c32(q[c] - q[c - stride], q[c + stride] - q[c]);
+    m = mm_c32(q[c] - q[c - stride], q[c + stride] - q[c]);
+    if (g->lim != ORC_LIM_MC) return m;
+    return mm_c32(0.5f * (q[c + stride] - q[c - stride]), g->theta * m);
 }
 
 /*
@@ -291,7 +304,7 @@ static float c32_face(const c32grid *g, float **qp, float **qm, const float *sgn
     } else {
         int cl = c0 + (k - 1) * stride, cr = cl + stride;
         G = p[cl] + m[cr];
-        if (g->order == 2) d = c32_slope(p, cl, k - 1, stride, len) - c32_slope(m, cr, k, stride, len);
+        if (g->order == 2) d = c32_slope(g, p, cl, k - 1, stride, len) - c32_slope(g, m, cr, k, stride, len);
     }
     /* second order: lam*(G + d/2) as one fused multiply-add (d = 0 at the walls) */
     return g->order == 2 ? fmaf(0.5f * lam, d, lam * G) : lam * G;
@@ -484,6 +497,26 @@ double orc_minmod(int mode, double l, double r)
     return mm_f64(l, r);
 }
 
+double orc_slope(int mode, int limiter, double theta, double ql, double qc, double qr)
+{
+    if (mode == ORC_REF32) {
+        grid_r32 g;
+        g.lim = limiter; g.theta = (float)theta;
+        return (double)slope_r32(&g, (float)ql, (float)qc, (float)qr);
+    }
+    if (mode == ORC_C32 || mode == ORC_C32K) {
+        c32grid g;
+        float q[3] = {(float)ql, (float)qc, (float)qr};
+        g.lim = limiter; g.theta = (float)theta;
+        return (double)c32_slope(&g, q, 1, 1, 1, 3);
+    }
+    {
+        grid_f64 g;
+        g.lim = limiter; g.theta = theta;
+        return slope_f64(&g, ql, qc, qr);
+    }
+}
+
 /* ------------------------------------------------------------------------- */
 /* Initial conditions.  Values pass through float exactly as the reference's  */
 /* float arrays hold them.                                                    */
@@ -541,6 +574,34 @@ void orc_ic_four_shocks(int nx, int ny, double *prim)
                 r = 1.5f; vx = 0.0f; vy = 0.0f; T = (float)(1.5 / (double)(r * R));
             }
             prim[c] = r; prim[n + c] = vx; prim[2 * n + c] = vy; prim[3 * n + c] = T;
+        }
+}
+
+/* base-omp/2nd_order_base_shll.c:149-167 ("Configuration 6", four contacts), with the
+ * cell counter starting at 0 (that file's reset of it sits in a commented-out block).
+ * Regions by the double comparisons of the reference; cells on the lines i == 0.5*NX or
+ * j == 0.5*NY fall through to the last branch.  T = 1.0/(rho*R) in double from the
+ * float product, stored as float. */
+void orc_ic_four_contacts(int nx, int ny, double *prim)
+{
+    int i, j;
+    size_t n = (size_t)nx * ny;
+    const float R = 1.0f;
+    for (i = 0; i < nx; i++)
+        for (j = 0; j < ny; j++) {
+            size_t c = (size_t)i * ny + j;
+            float r, vx, vy;
+            if ((i < 0.5 * nx) && (j < 0.5 * ny)) {
+                r = 1.0f; vx = -0.75f; vy = 0.5f;
+            } else if ((i > 0.5 * nx) && (j < 0.5 * ny)) {
+                r = 3.0f; vx = -0.75f; vy = -0.5f;
+            } else if ((i < 0.5 * nx) && (j > 0.5 * ny)) {
+                r = 2.0f; vx = 0.75f; vy = 0.5f;
+            } else {
+                r = 1.0f; vx = 0.75f; vy = -0.5f;
+            }
+            prim[c] = r; prim[n + c] = vx; prim[2 * n + c] = vy;
+            prim[3 * n + c] = (float)(1.0 / (double)(r * R));
         }
 }
 

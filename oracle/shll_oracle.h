@@ -10,6 +10,7 @@
  *   base-c/base_shll.c          (1D, first order, reflective walls)
  *   base-c/base_shll_2d.c       (2D, first order, reflective walls)
  *   base-c/2nd_order_base_shll.c(2D, second order minmod, outflow)
+ *   base-omp/2nd_order_base_shll.c (the same with the MC limiter, alpha = 1.25)
  * and a derived 1D second-order scheme (the x sweep of the 2D second-order
  * file on the 3-variable system, i.e. the 2D file run on y-uniform data with
  * v = 0).  It keeps the reference's unfused pass structure (one full-array
@@ -44,6 +45,11 @@ extern "C" {
 
 enum { ORC_F64 = 0, ORC_REF32 = 1, ORC_C32 = 2, ORC_C32K = 3 };
 enum { ORC_BC_REFLECT = 0, ORC_BC_OUTFLOW = 1 };
+/* Slope limiters of the second-order scheme:
+ *   ORC_LIM_MINMOD : minmod of the one-sided differences (base-c/2nd_order_base_shll.c:268-276)
+ *   ORC_LIM_MC     : minmod(central difference / 2, theta * minmod(one-sided differences)),
+ *                    the monotonized-central family (base-omp/2nd_order_base_shll.c:317-325,385-392) */
+enum { ORC_LIM_MINMOD = 0, ORC_LIM_MC = 1 };
 
 typedef struct {
     int32_t dim;    /* 1 or 2 */
@@ -51,9 +57,10 @@ typedef struct {
     int32_t bc;     /* ORC_BC_REFLECT (first-order files) / ORC_BC_OUTFLOW (second-order file) */
     int32_t nx;     /* cells in x */
     int32_t ny;     /* cells in y (1 in 1D) */
-    int32_t pad;
+    int32_t limiter;/* second order: ORC_LIM_MINMOD or ORC_LIM_MC */
     double lx;      /* domain length in x (reference: L = 1) */
     double ly;      /* domain length in y (reference: H = 1) */
+    double theta;   /* ORC_LIM_MC weight (base-omp/2nd_order_base_shll.c:44: alpha = 1.25) */
 } orc_problem;
 
 /* Number of variables: 3 in 1D (rho, rho*u, E), 4 in 2D (rho, rho*u, rho*v, E). */
@@ -96,6 +103,8 @@ int orc_stage(const orc_problem *pb, int mode, const double *prim_in, double *co
 
 /* minmod of the second-order file, in float (REF32) or double (F64). */
 double orc_minmod(int mode, double l, double r);
+/* Limited slope of an interior cell from the values q at c-1, c, c+1 (ORC_LIM_*). */
+double orc_slope(int mode, int limiter, double theta, double ql, double qc, double qr);
 
 /* ---- initial conditions (primitive SoA doubles, exact floats) ---- */
 /* Sod tube: cells with i < x0*N get (rhoL,uL,TL) else (rhoR,uR,TR).
@@ -106,6 +115,8 @@ void orc_ic_tube(int n, double x0, double rhoL, double uL, double TL,
 void orc_ic_implosion(int nx, int ny, double *prim);
 /* Four shocks of 2nd_order_base_shll.c:133-148. */
 void orc_ic_four_shocks(int nx, int ny, double *prim);
+/* Four contacts ("Configuration 6") of base-omp/2nd_order_base_shll.c:149-167. */
+void orc_ic_four_contacts(int nx, int ny, double *prim);
 /* Circle: rho_in for (x-cx)^2+(y-cy)^2 < r^2 at cell centres ((i+.5)/nx*lx,(j+.5)/ny*ly). */
 void orc_ic_circle(int nx, int ny, double lx, double ly, double cx, double cy, double r,
                    double rho_in, double T_in, double rho_out, double T_out, double *prim);

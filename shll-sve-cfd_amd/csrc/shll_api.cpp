@@ -20,9 +20,9 @@
 #include "shll_types.h"
 
 namespace shll {
-hipError_t launch_step1d(int prec, int order, int C, int wpb, int n_items, hipStream_t s, const StepArgs& a);
+hipError_t launch_step1d(int prec, int order, int lim, int C, int wpb, int n_items, hipStream_t s, const StepArgs& a);
 bool step1d_supported(int C, int wpb);
-hipError_t launch_step2d(int prec, int order, int wpb, int n_items, hipStream_t s, const StepArgs& a);
+hipError_t launch_step2d(int prec, int order, int lim, int wpb, int n_items, hipStream_t s, const StepArgs& a);
 hipError_t launch_init(int prec, int dim, int nblocks, int n_inst, hipStream_t st, const Inst* insts,
                        const double* prim, void* o0, void* o1, int64_t plane);
 hipError_t launch_output(int prec, int dim, int nblocks, int n_inst, hipStream_t st, const Inst* insts,
@@ -68,7 +68,7 @@ extern "C" const char* shll_version(void) { return "shll_amd 0.1 (gfx950)"; }
 static int ke_of(int dim, int order) { return (dim == 1 && order == 1) ? 0 : 1; }
 
 // base_shll.c:17-22,83-84 (float) and the same in double; see Consts.
-static Consts make_consts(int64_t nx, int64_t ny, double lx, double ly) {
+static Consts make_consts(int64_t nx, int64_t ny, double lx, double ly, double theta) {
     Consts k;
     std::memset(&k, 0, sizeof(k));
     {
@@ -97,6 +97,10 @@ static Consts make_consts(int64_t nx, int64_t ny, double lx, double ly) {
     k.c_R = (float)k.d_R;
     k.c_lamx = (float)k.d_lamx;
     k.c_lamy = (float)k.d_lamy;
+    // MC weight: float in the reference (base-omp/2nd_order_base_shll.c:44)
+    k.d_theta = theta;
+    k.f_theta = (float)theta;
+    k.c_theta = (float)theta;
     return k;
 }
 
@@ -126,6 +130,12 @@ static shll_status check_problem(const shll_problem* pb) {
     if (pb->dim == 1 && pb->ny != 1) return fail(SHLL_EINVAL, "ny must be 1 in 1D");
     if (pb->dim == 2 && (pb->ny < 2 || pb->ny > (1 << 30))) return fail(SHLL_EINVAL, "ny out of range");
     if (pb->dim == 2 && (double)pb->nx * (double)pb->ny > 4.0e9) return fail(SHLL_EINVAL, "grid too large");
+    if (pb->limiter != SHLL_LIM_MINMOD && pb->limiter != SHLL_LIM_MC)
+        return fail(SHLL_EINVAL, "bad limiter %d", pb->limiter);
+    if (pb->limiter == SHLL_LIM_MC && pb->order != 2)
+        return fail(SHLL_EINVAL, "the MC limiter needs order 2");
+    if (pb->limiter == SHLL_LIM_MC && !(pb->theta >= 1.0 && pb->theta <= 2.0))
+        return fail(SHLL_EINVAL, "MC limiter theta must lie in [1, 2] (got %g)", pb->theta);
     if (!(pb->lx > 0.0) || !std::isfinite(pb->lx)) return fail(SHLL_EINVAL, "lx must be > 0");
     if (pb->dim == 2 && (!(pb->ly > 0.0) || !std::isfinite(pb->ly))) return fail(SHLL_EINVAL, "ly must be > 0");
     return SHLL_OK;
@@ -207,7 +217,7 @@ static void tile2d(int32_t inst, int64_t nx, int64_t ny, int order, int rows, st
 // the engine shared by contexts (one instance) and batches (many)
 // ---------------------------------------------------------------------------
 struct Engine {
-    int dim = 1, order = 1, prec = 0, nv = 3;
+    int dim = 1, order = 1, prec = 0, nv = 3, lim = 0;
     int device = 0;
     hipStream_t stream = nullptr;
     int64_t plane = 0;                 // cells per variable plane
@@ -389,8 +399,8 @@ static StepArgs make_args(const Engine* E, int64_t j, int parity, int64_t n_item
 }
 
 static hipError_t launch_one(const Engine* E, const StepArgs& a, int64_t n_items) {
-    if (E->dim == 1) return launch_step1d(E->prec, E->order, E->L.C, E->L.wpb, (int)n_items, E->stream, a);
-    return launch_step2d(E->prec, E->order, E->L.wpb, (int)n_items, E->stream, a);
+    if (E->dim == 1) return launch_step1d(E->prec, E->order, E->lim, E->L.C, E->L.wpb, (int)n_items, E->stream, a);
+    return launch_step2d(E->prec, E->order, E->lim, E->L.wpb, (int)n_items, E->stream, a);
 }
 
 // Single instance: every launch has the same items; ksteps varies only on the last one.
@@ -679,11 +689,12 @@ struct shll_ctx {
     Engine* E;
 };
 
-static Engine* engine_new(int dim, int order, int prec) {
+static Engine* engine_new(int dim, int order, int prec, int lim) {
     Engine* E = new Engine();
     E->dim = dim;
     E->order = order;
     E->prec = prec;
+    E->lim = lim;
     E->nv = dim == 1 ? 3 : 4;
     return E;
 }
@@ -719,7 +730,7 @@ extern "C" shll_status shll_init(const shll_problem* pb, const double* prim_soa,
     if (st != SHLL_OK) return st;
     if (!prim_soa) return fail(SHLL_EINVAL, "prim_soa is NULL");
     if ((st = ensure_device()) != SHLL_OK) return st;
-    Engine* E = engine_new(pb->dim, pb->order, pb->precision);
+    Engine* E = engine_new(pb->dim, pb->order, pb->precision, pb->limiter);
     if ((st = engine_create_stream(E)) != SHLL_OK) { engine_free(E); return st; }
     Inst in;
     std::memset(&in, 0, sizeof(in));
@@ -729,7 +740,7 @@ extern "C" shll_status shll_init(const shll_problem* pb, const double* prim_soa,
     in.ny = (int32_t)pb->ny;
     in.bc = pb->bc;
     in.ke = ke_of(pb->dim, pb->order);
-    in.k = make_consts(pb->nx, pb->ny, pb->lx, pb->dim == 2 ? pb->ly : 1.0);
+    in.k = make_consts(pb->nx, pb->ny, pb->lx, pb->dim == 2 ? pb->ly : 1.0, pb->theta);
     E->h_inst.push_back(in);
     const int64_t cells = pb->nx * pb->ny;
     E->inst_cells.push_back(cells);
@@ -1043,8 +1054,8 @@ static std::string tune_key(const Engine* E, int64_t shape_class) {
     std::string arch = "unknown";
     if (hipGetDeviceProperties(&prop, E->device) == hipSuccess) arch = prop.gcnArchName;
     char buf[256];
-    std::snprintf(buf, sizeof(buf), "%s dim%d order%d prec%d batch%d class%lld", arch.c_str(), E->dim, E->order, E->prec,
-                  E->batch ? 1 : 0, (long long)shape_class);
+    std::snprintf(buf, sizeof(buf), "%s dim%d order%d%s prec%d batch%d class%lld", arch.c_str(), E->dim, E->order,
+                  E->lim == SHLL_LIM_MC ? "mc" : "", E->prec, E->batch ? 1 : 0, (long long)shape_class);
     return buf;
 }
 
@@ -1175,13 +1186,13 @@ extern "C" shll_status shll_batch_init(const shll_problem* items, const double* 
     for (int32_t i = 0; i < n; ++i) {
         if ((st = check_problem(&items[i])) != SHLL_OK) return st;
         if (items[i].dim != items[0].dim || items[i].order != items[0].order ||
-            items[i].precision != items[0].precision)
-            return fail(SHLL_EINVAL, "batch instances must share dim, order and precision");
+            items[i].precision != items[0].precision || items[i].limiter != items[0].limiter)
+            return fail(SHLL_EINVAL, "batch instances must share dim, order, precision and limiter");
         if (!prims[i]) return fail(SHLL_EINVAL, "prims[%d] is NULL", i);
         if (nsteps[i] < 0) return fail(SHLL_EINVAL, "nsteps[%d] < 0", i);
     }
     if ((st = ensure_device()) != SHLL_OK) return st;
-    Engine* E = engine_new(items[0].dim, items[0].order, items[0].precision);
+    Engine* E = engine_new(items[0].dim, items[0].order, items[0].precision, items[0].limiter);
     E->batch = true;
     if ((st = engine_create_stream(E)) != SHLL_OK) { engine_free(E); return st; }
     int64_t off = 0, maxcells = 0, total = 0;
@@ -1194,7 +1205,8 @@ extern "C" shll_status shll_batch_init(const shll_problem* items, const double* 
         in.ny = (int32_t)items[i].ny;
         in.bc = items[i].bc;
         in.ke = ke_of(items[i].dim, items[i].order);
-        in.k = make_consts(items[i].nx, items[i].ny, items[i].lx, items[i].dim == 2 ? items[i].ly : 1.0);
+        in.k = make_consts(items[i].nx, items[i].ny, items[i].lx, items[i].dim == 2 ? items[i].ly : 1.0,
+                           items[i].theta);
         const int64_t cells = items[i].nx * items[i].ny;
         E->h_inst.push_back(in);
         E->inst_cells.push_back(cells);
@@ -1406,6 +1418,31 @@ extern "C" void shll_ic_four_shocks(int64_t nx, int64_t ny, double* prim) {
             }
             prim[c] = r; prim[n + c] = vx; prim[2 * n + c] = vy; prim[3 * n + c] = T;
         }
+}
+
+// base-omp/2nd_order_base_shll.c:149-167 ("Configuration 6"): four quadrants around the
+// centre, split by the reference's double tests i < 0.5*NX, i > 0.5*NX, j < 0.5*NY,
+// j > 0.5*NY (cells on a centre line take the last region); T = 1/(rho*R) rounded to float.
+extern "C" void shll_ic_four_contacts(int64_t nx, int64_t ny, double* prim) {
+    struct Region {
+        float rho, u, v;
+    };
+    static const Region lower_left = {1.0f, -0.75f, 0.5f}, lower_right = {3.0f, -0.75f, -0.5f},
+                        upper_left = {2.0f, 0.75f, 0.5f}, rest = {1.0f, 0.75f, -0.5f};
+    const int64_t n = nx * ny;
+    const float R = 1.0f;
+    for (int64_t i = 0; i < nx; ++i) {
+        const bool ilo = i < 0.5 * nx, ihi = i > 0.5 * nx;
+        for (int64_t j = 0; j < ny; ++j) {
+            const bool jlo = j < 0.5 * ny, jhi = j > 0.5 * ny;
+            const Region& g = (ilo && jlo) ? lower_left : (ihi && jlo) ? lower_right : (ilo && jhi) ? upper_left : rest;
+            const int64_t c = i * ny + j;
+            prim[c] = g.rho;
+            prim[n + c] = g.u;
+            prim[2 * n + c] = g.v;
+            prim[3 * n + c] = (float)(1.0 / (double)(g.rho * R));
+        }
+    }
 }
 
 extern "C" void shll_ic_circle(int64_t nx, int64_t ny, double lx, double ly, double cx, double cy, double r,

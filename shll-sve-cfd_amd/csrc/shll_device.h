@@ -1,11 +1,12 @@
 // shll_device.h -- device arithmetic of the SHLL step for gfx950.
 //
-// Three precisions share one set of templates:
+// The precisions share one set of templates:
 //   PREC_F64   : every value double, the reference association (FP64 oracle form)
 //   PREC_REF32 : float values with the reference's double promotions, so results are
 //                bitwise those of the reference C on x86-64
 //   PREC_C32   : float arithmetic, (hi, lo) float state, conservative face update
 //                with TwoSum (the FP32-compensated form)
+//   PREC_C32K  : PREC_C32 with one Kahan step per cell for the update
 //
 // The library is compiled with -ffp-contract=off: every + - * below is one IEEE-754
 // operation in source order (fused multiply-adds only where written), and division
@@ -156,6 +157,9 @@ struct RefArith {
         RT pick = fabs((double)l) < fabs((double)r) ? l : r;
         return (double)(l * r) < 0.0 ? (RT)0 : pick;
     }
+    // 0.5*(difference) with the double literal of base-omp/2nd_order_base_shll.c:317,
+    // rounded to RT as minmod's argument
+    __device__ static __forceinline__ RT half(RT x) { return (RT)(0.5 * (double)x); }
 };
 
 // FP32-compensated arithmetic: float, with explicit fused multiply-adds where a
@@ -191,7 +195,18 @@ struct C32Arith {
     // zero result is unspecified (a zero slope only ever meets a nonzero or +0 face
     // flux, where the sign of the zero cannot change the sum).
     __device__ static __forceinline__ float mm(float l, float r) { return __builtin_amdgcn_fmed3f(l, r, 0.0f); }
+    __device__ static __forceinline__ float half(float x) { return 0.5f * x; }
 };
+
+// Limited slope of a split flux from its values at c-1, c, c+1 (interior cells):
+//   LIM_MINMOD  mm(qc - ql, qr - qc)                                 2nd_order_base_shll.c:268-276
+//   LIM_MC      mm((qr - ql)/2, theta * mm(qc - ql, qr - qc))        base-omp/2nd_order_base_shll.c:317-325
+template <class A, int LIM, class RT>
+__device__ __forceinline__ RT limited_slope(RT ql, RT qc, RT qr, RT theta) {
+    const RT s = A::mm(qc - ql, qr - qc);
+    if constexpr (LIM == LIM_MINMOD) return s;
+    else return A::mm(A::half(qr - ql), theta * s);
+}
 
 // TwoSum (Knuth, TAOCP vol. 2, 4.2.2): s + e == a + b exactly
 __device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {

@@ -13,7 +13,7 @@
 //
 // References: base_shll.c:87-177 (first order, reflective), 2nd_order_base_shll.c
 // :203-459 (second order, outflow; its x sweep on the 3-variable system is the 1D
-// second-order scheme).
+// second-order scheme), base-omp/2nd_order_base_shll.c:317-325 (the MC limiter).
 #include "shll_device.h"
 
 namespace shll {
@@ -31,9 +31,9 @@ enum { WM_NONE = 0, WM_GHOST = 1, WM_RULES = 2 };
 
 // slope of a split-flux window at position p (window index p+2) of global cell gg;
 // with WM_RULES zero on the two boundary cells and beyond (2nd_order_base_shll.c:227-256)
-template <class A, int WM, class RT, int NW>
-__device__ __forceinline__ RT slope_at(const RT (&W)[NW], int p, int gg, int n) {
-    RT s = A::mm(W[p + 2] - W[p + 1], W[p + 3] - W[p + 2]);
+template <class A, int WM, int LIM, class RT, int NW>
+__device__ __forceinline__ RT slope_at(const RT (&W)[NW], int p, int gg, int n, RT theta) {
+    RT s = limited_slope<A, LIM>(W[p + 1], W[p + 2], W[p + 3], theta);
     if constexpr (WM != WM_RULES) return s;
     return (gg <= 0 || gg >= n - 1) ? (RT)0 : s;
 }
@@ -64,7 +64,8 @@ __device__ __forceinline__ void make_window(const RT (&q)[C], RT (&W)[C + 4]) {
 // F-(n) = F-(n-1) for outflow (2nd_order_base_shll.c:212-256), F+(-1) = s*F-(0) and
 // F-(n) = s*F+(n-1) with s = (-1, +1, -1) for reflection (base_shll.c:93-110).  With
 // copied ghosts every boundary slope also comes out zero through the limiter (one of
-// its differences is exactly 0), as the reference sets them; reflective second order
+// its differences is exactly 0, and so is then the MC form's second argument), as the
+// reference sets them; reflective second order
 // needs the explicit rule (WM_RULES).  Cells beyond the ghosts only feed the ghosts'
 // own updates, which are replaced again before the next step.
 template <int C, class RT>
@@ -93,13 +94,14 @@ __device__ __forceinline__ void ghost_cells(RT (&q)[3][C], int gl, int n, bool r
 // ---------------------------------------------------------------------------
 // F64 / REF32: the reference association, cell by cell
 // ---------------------------------------------------------------------------
-template <int PREC, int ORDER, int C, int WM>
+template <int PREC, int ORDER, int C, int WM, int LIM>
 __device__ __forceinline__ void step_ref(typename Real<PREC>::T (&u)[3][C], const Inst& in, int gl) {
     using RT = typename Real<PREC>::T;
     using A = RefArith<RT>;
     const RT cv = PREC == PREC_F64 ? (RT)in.k.d_cv : (RT)in.k.f_cv;
     const RT gR = PREC == PREC_F64 ? (RT)in.k.d_gR : (RT)in.k.f_gR;
     const RT lam = PREC == PREC_F64 ? (RT)in.k.d_lamx : (RT)in.k.f_lamx;
+    const RT th = PREC == PREC_F64 ? (RT)in.k.d_theta : (RT)in.k.f_theta;
     const int n = in.nx;
     const bool refl = in.bc == BC_REFLECT;
     if constexpr (WM != WM_NONE) ghost_cells<C>(u, gl, n, refl);
@@ -128,10 +130,10 @@ __device__ __forceinline__ void step_ref(typename Real<PREC>::T (&u)[3][C], cons
             if constexpr (ORDER == 2) {
                 // own slopes, the left neighbour's F+ slope and the right neighbour's F-
                 // slope; every slope of a boundary or ghost cell is zero
-                RT dp = slope_at<A, WM>(FP, m, g, n);
-                RT dm = slope_at<A, WM>(FM, m, g, n);
-                RT Ldf = slope_at<A, WM>(FP, m - 1, g - 1, n);
-                RT Rdf = slope_at<A, WM>(FM, m + 1, g + 1, n);
+                RT dp = slope_at<A, WM, LIM>(FP, m, g, n, th);
+                RT dm = slope_at<A, WM, LIM>(FM, m, g, n, th);
+                RT Ldf = slope_at<A, WM, LIM>(FP, m - 1, g - 1, n, th);
+                RT Rdf = slope_at<A, WM, LIM>(FM, m + 1, g + 1, n, th);
                 uu = (RT)((double)uu - 0.5 * (double)lam * (double)(dp + dm - Rdf - Ldf));
             }
             u[v][m] = uu;
@@ -144,10 +146,10 @@ __device__ __forceinline__ void step_ref(typename Real<PREC>::T (&u)[3][C], cons
 // ---------------------------------------------------------------------------
 // KAHAN: the face terms t[m] - t[m+1] enter each cell by one Kahan step (lo holds
 // the part of the value not yet in hi) instead of the exact TwoSum chain
-template <int ORDER, int C, int WM, bool KAHAN>
+template <int ORDER, int C, int WM, bool KAHAN, int LIM>
 __device__ __forceinline__ void step_c32(float (&hi)[3][C], float (&lo)[3][C], const Inst& in, int gl) {
     using A = C32Arith;
-    const float rcv = in.k.c_rcv, gR = in.k.c_gR, lam = in.k.c_lamx, hlam = 0.5f * lam;
+    const float rcv = in.k.c_rcv, gR = in.k.c_gR, lam = in.k.c_lamx, hlam = 0.5f * lam, th = in.k.c_theta;
     const int n = in.nx;
     if constexpr (WM != WM_NONE) ghost_cells<C>(hi, gl, n, in.bc == BC_REFLECT);
     float fp[3][C], fm[3][C];
@@ -172,7 +174,7 @@ __device__ __forceinline__ void step_c32(float (&hi)[3][C], float (&lo)[3][C], c
             const int gf = gl + f;  // face gf lies between cells gf-1 and gf
             const float G = FP[f + 1] + FM[f + 2];
             if constexpr (ORDER == 2) {
-                const float d = slope_at<A, WM>(FP, f - 1, gf - 1, n) - slope_at<A, WM>(FM, f, gf, n);
+                const float d = slope_at<A, WM, LIM>(FP, f - 1, gf - 1, n, th) - slope_at<A, WM, LIM>(FM, f, gf, n, th);
                 t[f] = __builtin_fmaf(hlam, d, lam * G);  // lam*(G + d/2)
             } else {
                 t[f] = lam * G;
@@ -202,10 +204,10 @@ __device__ __forceinline__ void step_c32(float (&hi)[3][C], float (&lo)[3][C], c
 // wall mode gets its own copy of loop and store, so the loop-carried state never
 // has to be moved between the copies' registers.
 // ---------------------------------------------------------------------------
-template <int ORDER, int C, int WM, bool KAHAN>
+template <int ORDER, int C, int WM, bool KAHAN, int LIM>
 __device__ __forceinline__ void advance_c32(float (&hi)[3][C], float (&lo)[3][C], const Inst& in, const Item1& it,
                                             int gl, int ns, const StepArgs& a) {
-    for (int s = 0; s < ns; ++s) step_c32<ORDER, C, WM, KAHAN>(hi, lo, in, gl);
+    for (int s = 0; s < ns; ++s) step_c32<ORDER, C, WM, KAHAN, LIM>(hi, lo, in, gl);
     float* __restrict__ oh = (float*)a.out0 + in.off;
     float* __restrict__ ol = (float*)a.out1 + in.off;
     const int64_t P = a.plane;
@@ -222,11 +224,11 @@ __device__ __forceinline__ void advance_c32(float (&hi)[3][C], float (&lo)[3][C]
     }
 }
 
-template <int PREC, int ORDER, int C, int WM>
+template <int PREC, int ORDER, int C, int WM, int LIM>
 __device__ __forceinline__ void advance_ref(typename Real<PREC>::T (&u)[3][C], const Inst& in, const Item1& it,
                                             int gl, int ns, const StepArgs& a) {
     using RT = typename Real<PREC>::T;
-    for (int s = 0; s < ns; ++s) step_ref<PREC, ORDER, C, WM>(u, in, gl);
+    for (int s = 0; s < ns; ++s) step_ref<PREC, ORDER, C, WM, LIM>(u, in, gl);
     RT* __restrict__ ou = (RT*)a.out0 + in.off;
     const int64_t P = a.plane;
 #pragma unroll
@@ -240,9 +242,9 @@ __device__ __forceinline__ void advance_ref(typename Real<PREC>::T (&u)[3][C], c
 }
 
 // ---------------------------------------------------------------------------
-// the launch: one wavefront per Item1
+// the launch: one wavefront per Item1 (LIM: the second-order limiter)
 // ---------------------------------------------------------------------------
-template <int PREC, int ORDER, int C, int WPB>
+template <int PREC, int ORDER, int C, int WPB, int LIM>
 __global__ __launch_bounds__(64 * WPB) void k_step1d(StepArgs a) {
     using RT = typename Real<PREC>::T;
     const int wave = (int)blockIdx.x * WPB + uniform((int)threadIdx.x >> 6);
@@ -277,9 +279,9 @@ __global__ __launch_bounds__(64 * WPB) void k_step1d(StepArgs a) {
                 lo[v][m] = ok ? il[v * P + g] : 0.0f;
             }
         }
-        if (wm == WM_NONE) advance_c32<ORDER, C, WM_NONE, K>(hi, lo, in, it, gl, ns, a);
-        else if (ORDER == 1 || wm == WM_GHOST) advance_c32<ORDER, C, WM_GHOST, K>(hi, lo, in, it, gl, ns, a);
-        else advance_c32<ORDER, C, WR, K>(hi, lo, in, it, gl, ns, a);
+        if (wm == WM_NONE) advance_c32<ORDER, C, WM_NONE, K, LIM>(hi, lo, in, it, gl, ns, a);
+        else if (ORDER == 1 || wm == WM_GHOST) advance_c32<ORDER, C, WM_GHOST, K, LIM>(hi, lo, in, it, gl, ns, a);
+        else advance_c32<ORDER, C, WR, K, LIM>(hi, lo, in, it, gl, ns, a);
     } else {
         const RT* __restrict__ iu = (const RT*)a.in0 + in.off;
         RT u[3][C];
@@ -290,47 +292,56 @@ __global__ __launch_bounds__(64 * WPB) void k_step1d(StepArgs a) {
 #pragma unroll
             for (int v = 0; v < 3; ++v) u[v][m] = ok ? iu[v * P + g] : (RT)(v == 0 ? 1.0 : (v == 1 ? 0.0 : 2.5));
         }
-        if (wm == WM_NONE) advance_ref<PREC, ORDER, C, WM_NONE>(u, in, it, gl, ns, a);
-        else if (ORDER == 1 || wm == WM_GHOST) advance_ref<PREC, ORDER, C, WM_GHOST>(u, in, it, gl, ns, a);
-        else advance_ref<PREC, ORDER, C, WR>(u, in, it, gl, ns, a);
+        if (wm == WM_NONE) advance_ref<PREC, ORDER, C, WM_NONE, LIM>(u, in, it, gl, ns, a);
+        else if (ORDER == 1 || wm == WM_GHOST) advance_ref<PREC, ORDER, C, WM_GHOST, LIM>(u, in, it, gl, ns, a);
+        else advance_ref<PREC, ORDER, C, WR, LIM>(u, in, it, gl, ns, a);
     }
 }
 
 // ---------------------------------------------------------------------------
 // dispatch
 // ---------------------------------------------------------------------------
-template <int PREC, int ORDER, int C>
+template <int PREC, int ORDER, int C, int LIM>
 static hipError_t launch_c(int wpb, int nblocks, hipStream_t s, const StepArgs& a) {
     if (wpb == 1)
-        hipLaunchKernelGGL((k_step1d<PREC, ORDER, C, 1>), dim3(nblocks), dim3(64), 0, s, a);
+        hipLaunchKernelGGL((k_step1d<PREC, ORDER, C, 1, LIM>), dim3(nblocks), dim3(64), 0, s, a);
     else
-        hipLaunchKernelGGL((k_step1d<PREC, ORDER, C, 4>), dim3(nblocks), dim3(256), 0, s, a);
+        hipLaunchKernelGGL((k_step1d<PREC, ORDER, C, 4, LIM>), dim3(nblocks), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
-template <int PREC, int ORDER>
+template <int PREC, int ORDER, int LIM>
 static hipError_t launch_o(int C, int wpb, int nblocks, hipStream_t s, const StepArgs& a) {
     switch (C) {
-    case 1: return launch_c<PREC, ORDER, 1>(wpb, nblocks, s, a);
-    case 2: return launch_c<PREC, ORDER, 2>(wpb, nblocks, s, a);
-    case 4: return launch_c<PREC, ORDER, 4>(wpb, nblocks, s, a);
-    case 8: return launch_c<PREC, ORDER, 8>(wpb, nblocks, s, a);
+    case 1: return launch_c<PREC, ORDER, 1, LIM>(wpb, nblocks, s, a);
+    case 2: return launch_c<PREC, ORDER, 2, LIM>(wpb, nblocks, s, a);
+    case 4: return launch_c<PREC, ORDER, 4, LIM>(wpb, nblocks, s, a);
+    case 8: return launch_c<PREC, ORDER, 8, LIM>(wpb, nblocks, s, a);
     default: return hipErrorInvalidValue;
     }
 }
 
-hipError_t launch_step1d(int prec, int order, int C, int wpb, int n_items, hipStream_t s, const StepArgs& a) {
+// first order; second order with minmod; second order with the MC limiter
+template <int PREC>
+static hipError_t launch_p(int order, int lim, int C, int wpb, int nblocks, hipStream_t s, const StepArgs& a) {
+    if (order == 1) return lim == LIM_MINMOD ? launch_o<PREC, 1, LIM_MINMOD>(C, wpb, nblocks, s, a) : hipErrorInvalidValue;
+    if (order != 2) return hipErrorInvalidValue;
+    if (lim == LIM_MINMOD) return launch_o<PREC, 2, LIM_MINMOD>(C, wpb, nblocks, s, a);
+    if (lim == LIM_MC) return launch_o<PREC, 2, LIM_MC>(C, wpb, nblocks, s, a);
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_step1d(int prec, int order, int lim, int C, int wpb, int n_items, hipStream_t s,
+                         const StepArgs& a) {
     const int nblocks = (n_items + wpb - 1) / wpb;
     if (nblocks <= 0) return hipSuccess;
-    if (prec == PREC_F64) return order == 1 ? launch_o<PREC_F64, 1>(C, wpb, nblocks, s, a)
-                                            : launch_o<PREC_F64, 2>(C, wpb, nblocks, s, a);
-    if (prec == PREC_REF32) return order == 1 ? launch_o<PREC_REF32, 1>(C, wpb, nblocks, s, a)
-                                              : launch_o<PREC_REF32, 2>(C, wpb, nblocks, s, a);
-    if (prec == PREC_C32) return order == 1 ? launch_o<PREC_C32, 1>(C, wpb, nblocks, s, a)
-                                            : launch_o<PREC_C32, 2>(C, wpb, nblocks, s, a);
-    if (prec == PREC_C32K) return order == 1 ? launch_o<PREC_C32K, 1>(C, wpb, nblocks, s, a)
-                                             : launch_o<PREC_C32K, 2>(C, wpb, nblocks, s, a);
-    return hipErrorInvalidValue;
+    switch (prec) {
+    case PREC_F64: return launch_p<PREC_F64>(order, lim, C, wpb, nblocks, s, a);
+    case PREC_REF32: return launch_p<PREC_REF32>(order, lim, C, wpb, nblocks, s, a);
+    case PREC_C32: return launch_p<PREC_C32>(order, lim, C, wpb, nblocks, s, a);
+    case PREC_C32K: return launch_p<PREC_C32K>(order, lim, C, wpb, nblocks, s, a);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 bool step1d_supported(int C, int wpb) { return (C == 1 || C == 2 || C == 4 || C == 8) && (wpb == 1 || wpb == 4); }

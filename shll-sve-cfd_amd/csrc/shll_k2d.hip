@@ -12,7 +12,8 @@
 // The update keeps the reference's unsplit order: x term, (second-order x term),
 // y term, (second-order y term), all from the same state (base_shll_2d.c:225-236,
 // 2nd_order_base_shll.c:436-458).  Walls: reflective with per-component signs
-// (base_shll_2d.c:150-216) or outflow (2nd_order_base_shll.c:214-323).
+// (base_shll_2d.c:150-216) or outflow (2nd_order_base_shll.c:214-323).  Second-order
+// slopes: minmod or the MC limiter of base-omp/2nd_order_base_shll.c:317-325,385-392.
 #include "shll_device.h"
 
 #ifndef PF_DEPTH
@@ -71,7 +72,7 @@ __device__ __forceinline__ void fluxes2(const RT (&U)[4], const Inst& in, RT (&x
 __device__ __forceinline__ int sgn_x(int v) { return v == 1 ? 1 : -1; }
 __device__ __forceinline__ int sgn_y(int v) { return v == 2 ? 1 : -1; }
 
-template <int PREC, int ORDER, int WPB>
+template <int PREC, int ORDER, int WPB, int LIM>
 __global__ __launch_bounds__(64 * WPB) void k_step2d(StepArgs a) {
     using RT = typename Real2<PREC>::T;
     using A = typename std::conditional<PREC == PREC_C32 || PREC == PREC_C32K, C32Arith, RefArith<RT>>::type;
@@ -114,6 +115,7 @@ __global__ __launch_bounds__(64 * WPB) void k_step2d(StepArgs a) {
 
     const RT lamx = PREC == PREC_F64 ? (RT)in.k.d_lamx : (PREC == PREC_REF32 ? (RT)in.k.f_lamx : (RT)in.k.c_lamx);
     const RT lamy = PREC == PREC_F64 ? (RT)in.k.d_lamy : (PREC == PREC_REF32 ? (RT)in.k.f_lamy : (RT)in.k.c_lamy);
+    const RT th = PREC == PREC_F64 ? (RT)in.k.d_theta : (PREC == PREC_REF32 ? (RT)in.k.f_theta : (RT)in.k.c_theta);
 
     // The state of row t+D is loaded while row t is processed (before row t's stores:
     // one counter tracks loads and stores, so a row's wait never covers the stores of
@@ -175,12 +177,14 @@ __global__ __launch_bounds__(64 * WPB) void k_step2d(StepArgs a) {
             // slopes (second order); zero on boundary rows / columns
             RT dfp = 0, dfm = 0, dfpl = 0, dfmr = 0, dhp = 0, dhm = 0, dhpl = 0, dhmr = 0;
             if constexpr (ORDER == 2) {
-                dfp = (c <= 0 || c >= nx - 1) ? (RT)0 : A::mm(xp[2][v] - xp[1][v], xp[3][v] - xp[2][v]);
-                dfm = (c <= 0 || c >= nx - 1) ? (RT)0 : A::mm(xm[2][v] - xm[1][v], xm[3][v] - xm[2][v]);
-                dfpl = (c - 1 <= 0 || c - 1 >= nx - 1) ? (RT)0 : A::mm(xp[1][v] - xp[0][v], xp[2][v] - xp[1][v]);
-                dfmr = (c + 1 <= 0 || c + 1 >= nx - 1) ? (RT)0 : A::mm(xm[3][v] - xm[2][v], xm[4][v] - xm[3][v]);
-                dhp = (j <= 0 || j >= ny - 1) ? (RT)0 : A::mm(hpc - hpl, hpr - hpc);
-                dhm = (j <= 0 || j >= ny - 1) ? (RT)0 : A::mm(hmc - hml, hmr - hmc);
+                dfp = (c <= 0 || c >= nx - 1) ? (RT)0 : limited_slope<A, LIM>(xp[1][v], xp[2][v], xp[3][v], th);
+                dfm = (c <= 0 || c >= nx - 1) ? (RT)0 : limited_slope<A, LIM>(xm[1][v], xm[2][v], xm[3][v], th);
+                dfpl = (c - 1 <= 0 || c - 1 >= nx - 1) ? (RT)0
+                                                       : limited_slope<A, LIM>(xp[0][v], xp[1][v], xp[2][v], th);
+                dfmr = (c + 1 <= 0 || c + 1 >= nx - 1) ? (RT)0
+                                                       : limited_slope<A, LIM>(xm[2][v], xm[3][v], xm[4][v], th);
+                dhp = (j <= 0 || j >= ny - 1) ? (RT)0 : limited_slope<A, LIM>(hpl, hpc, hpr, th);
+                dhm = (j <= 0 || j >= ny - 1) ? (RT)0 : limited_slope<A, LIM>(hml, hmc, hmr, th);
                 dhpl = from_left(dhp);
                 dhmr = from_right(dhm);
             }
@@ -253,25 +257,33 @@ __global__ __launch_bounds__(64 * WPB) void k_step2d(StepArgs a) {
     }
 }
 
-template <int PREC, int ORDER>
+template <int PREC, int ORDER, int LIM>
 static hipError_t launch2_o(int wpb, int nblocks, hipStream_t s, const StepArgs& a) {
-    if (wpb == 1) hipLaunchKernelGGL((k_step2d<PREC, ORDER, 1>), dim3(nblocks), dim3(64), 0, s, a);
-    else hipLaunchKernelGGL((k_step2d<PREC, ORDER, 4>), dim3(nblocks), dim3(256), 0, s, a);
+    if (wpb == 1) hipLaunchKernelGGL((k_step2d<PREC, ORDER, 1, LIM>), dim3(nblocks), dim3(64), 0, s, a);
+    else hipLaunchKernelGGL((k_step2d<PREC, ORDER, 4, LIM>), dim3(nblocks), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
-hipError_t launch_step2d(int prec, int order, int wpb, int n_items, hipStream_t s, const StepArgs& a) {
+// first order; second order with minmod; second order with the MC limiter
+template <int PREC>
+static hipError_t launch2_p(int order, int lim, int wpb, int nblocks, hipStream_t s, const StepArgs& a) {
+    if (order == 1) return lim == LIM_MINMOD ? launch2_o<PREC, 1, LIM_MINMOD>(wpb, nblocks, s, a) : hipErrorInvalidValue;
+    if (order != 2) return hipErrorInvalidValue;
+    if (lim == LIM_MINMOD) return launch2_o<PREC, 2, LIM_MINMOD>(wpb, nblocks, s, a);
+    if (lim == LIM_MC) return launch2_o<PREC, 2, LIM_MC>(wpb, nblocks, s, a);
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_step2d(int prec, int order, int lim, int wpb, int n_items, hipStream_t s, const StepArgs& a) {
     const int nblocks = (n_items + wpb - 1) / wpb;
     if (nblocks <= 0) return hipSuccess;
-    if (prec == PREC_F64) return order == 1 ? launch2_o<PREC_F64, 1>(wpb, nblocks, s, a)
-                                            : launch2_o<PREC_F64, 2>(wpb, nblocks, s, a);
-    if (prec == PREC_REF32) return order == 1 ? launch2_o<PREC_REF32, 1>(wpb, nblocks, s, a)
-                                              : launch2_o<PREC_REF32, 2>(wpb, nblocks, s, a);
-    if (prec == PREC_C32) return order == 1 ? launch2_o<PREC_C32, 1>(wpb, nblocks, s, a)
-                                            : launch2_o<PREC_C32, 2>(wpb, nblocks, s, a);
-    if (prec == PREC_C32K) return order == 1 ? launch2_o<PREC_C32K, 1>(wpb, nblocks, s, a)
-                                             : launch2_o<PREC_C32K, 2>(wpb, nblocks, s, a);
-    return hipErrorInvalidValue;
+    switch (prec) {
+    case PREC_F64: return launch2_p<PREC_F64>(order, lim, wpb, nblocks, s, a);
+    case PREC_REF32: return launch2_p<PREC_REF32>(order, lim, wpb, nblocks, s, a);
+    case PREC_C32: return launch2_p<PREC_C32>(order, lim, wpb, nblocks, s, a);
+    case PREC_C32K: return launch2_p<PREC_C32K>(order, lim, wpb, nblocks, s, a);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 }  // namespace shll

@@ -8,16 +8,20 @@ namespace shll {
 
 enum Prec : int { PREC_F64 = 0, PREC_REF32 = 1, PREC_C32 = 2, PREC_C32K = 3 };
 enum Bc : int { BC_REFLECT = 0, BC_OUTFLOW = 1 };
+// second-order slope limiters: minmod of the one-sided differences
+// (2nd_order_base_shll.c:268-276) or the MC form minmod(half central difference,
+// theta * minmod(one-sided differences)) (base-omp/2nd_order_base_shll.c:317-325)
+enum Lim : int { LIM_MINMOD = 0, LIM_MC = 1 };
 
 // Per-instance scheme constants, computed on the host (never folded on the device):
 //   F64   : d_*  (gamma = 1.4, CV = R/(gamma-1), lam = DT/DX)
 //   REF32 : f_*  (the reference's float constants, base_shll.c:17-22,83-84)
 //   C32   : c_*  (the F64 constants rounded once to float; rcv = 1/CV)
+// theta: the MC limiter's weight (the reference's float alpha = 1.25 in REF32)
 struct Consts {
-    double d_cv, d_gR, d_R, d_lamx, d_lamy;
-    float f_cv, f_gR, f_R, f_lamx, f_lamy;
-    float c_rcv, c_gR, c_R, c_lamx, c_lamy;
-    float pad0;
+    double d_cv, d_gR, d_R, d_lamx, d_lamy, d_theta;
+    float f_cv, f_gR, f_R, f_lamx, f_lamy, f_theta;
+    float c_rcv, c_gR, c_R, c_lamx, c_lamy, c_theta;
 };
 
 // One instance (tube or 2D domain) of a launch.

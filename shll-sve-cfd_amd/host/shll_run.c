@@ -1,19 +1,24 @@
 /*
  * shll_run.c -- C host driver: the reference's main() (base_shll.c:198-225,
- * base_shll_2d.c:343-370, 2nd_order_base_shll.c:565-592) over the GPU library.
+ * base_shll_2d.c:343-370, 2nd_order_base_shll.c:565-592, base-omp/2nd_order_base_shll.c
+ * :615-658) over the GPU library.
  *
  *   Allocate_and_Init_Memory; Compute_U_from_P; Compute_P_from_U   -> shll_init
  *   while (time < TOTAL_TIME) { F; Update; P; time += DT; }         -> shll_run_to_time
  *   printf("Completed in %d steps\n", NO_STEPS)                    -> same line
  *   Save_Results                                                    -> shll_write_results
+ *                                        (ten columns, --full)      -> shll_write_results_full
  *   Free_Memory                                                     -> shll_free
  *
  * The reference's compile-time constants become flags.  Defaults reproduce
  * base_shll.c (1D, N=256, t=0.2, Sod, reflective walls, reference float arithmetic).
  *
  *   shll_run [--dim 1|2] [--order 1|2] [--nx N] [--ny N] [--precision f64|ref32|c32|c32k]
- *            [--bc reflect|outflow] [--ic sod|implosion|four-shocks|circle]
- *            [--t-end T] [--steps K] [--out results.dat] [--sums]
+ *            [--bc reflect|outflow] [--ic sod|implosion|four-shocks|four-contacts|circle]
+ *            [--limiter minmod|mc] [--theta A] [--t-end T] [--steps K] [--out results.dat]
+ *            [--full] [--sums]
+ * --limiter mc (theta 1.25) with --dim 2 --order 2 --nx 1024 is base-omp/2nd_order_base_shll.c:
+ * its default initial data are the four contacts and its TOTAL_TIME 0.3 (:43).
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -25,8 +30,9 @@ static void usage(void)
 {
     fprintf(stderr,
             "usage: shll_run [--dim 1|2] [--order 1|2] [--nx N] [--ny N] [--precision f64|ref32|c32|c32k]\n"
-            "                [--bc reflect|outflow] [--ic sod|implosion|four-shocks|circle]\n"
-            "                [--t-end T] [--steps K] [--out PATH] [--sums]\n");
+            "                [--bc reflect|outflow] [--ic sod|implosion|four-shocks|four-contacts|circle]\n"
+            "                [--limiter minmod|mc] [--theta A] [--t-end T] [--steps K] [--out PATH]\n"
+            "                [--full] [--sums]\n");
 }
 
 int main(int argc, char **argv)
@@ -35,7 +41,7 @@ int main(int argc, char **argv)
     const char *ic = NULL, *out = "results.dat";
     float t_end = -1.0f;
     long long steps = -1;
-    int sums = 0, i, order_set = 0, bc_set = 0, nv;
+    int sums = 0, full = 0, i, order_set = 0, bc_set = 0, nv;
     double *prim;
     shll_ctx *ctx = NULL;
     shll_status st;
@@ -44,9 +50,11 @@ int main(int argc, char **argv)
     memset(&pb, 0, sizeof(pb));
     pb.dim = 1; pb.order = 1; pb.bc = SHLL_BC_REFLECT; pb.precision = SHLL_REF32;
     pb.nx = 256; pb.ny = 1; pb.lx = 1.0; pb.ly = 1.0;
+    pb.limiter = SHLL_LIM_MINMOD; pb.theta = 1.25;
     for (i = 1; i < argc; i++) {
         const char *a = argv[i], *v = i + 1 < argc ? argv[i + 1] : NULL;
         if (!strcmp(a, "--sums")) { sums = 1; continue; }
+        if (!strcmp(a, "--full")) { full = 1; continue; }
         if (!v) { usage(); return 2; }
         if (!strcmp(a, "--dim")) pb.dim = atoi(v);
         else if (!strcmp(a, "--order")) { pb.order = atoi(v); order_set = 1; }
@@ -59,6 +67,8 @@ int main(int argc, char **argv)
                                                 : SHLL_REF32;
         else if (!strcmp(a, "--bc")) { pb.bc = !strcmp(v, "outflow") ? SHLL_BC_OUTFLOW : SHLL_BC_REFLECT; bc_set = 1; }
         else if (!strcmp(a, "--ic")) ic = v;
+        else if (!strcmp(a, "--limiter")) pb.limiter = !strcmp(v, "mc") ? SHLL_LIM_MC : SHLL_LIM_MINMOD;
+        else if (!strcmp(a, "--theta")) pb.theta = atof(v);
         else if (!strcmp(a, "--t-end")) t_end = (float)atof(v);
         else if (!strcmp(a, "--steps")) steps = atoll(v);
         else if (!strcmp(a, "--out")) out = v;
@@ -68,9 +78,10 @@ int main(int argc, char **argv)
     (void)order_set;
     if (pb.dim == 2 && pb.ny == 1) pb.ny = pb.nx;
     if (!bc_set) pb.bc = pb.order == 1 ? SHLL_BC_REFLECT : SHLL_BC_OUTFLOW;
-    if (!ic) ic = pb.dim == 1 ? "sod" : (pb.order == 1 ? "implosion" : "four-shocks");
-    /* TOTAL_TIME of each reference file (base_shll.c:26, base_shll_2d.c:35, 2nd_order_base_shll.c:43) */
-    if (t_end < 0) t_end = pb.dim == 1 ? 0.2f : (pb.order == 1 ? 0.1f : 0.8f);
+    if (!ic) ic = pb.dim == 1 ? "sod" : (pb.order == 1 ? "implosion" : (pb.limiter == SHLL_LIM_MC ? "four-contacts" : "four-shocks"));
+    /* TOTAL_TIME of each reference file (base_shll.c:26, base_shll_2d.c:35, 2nd_order_base_shll.c:43,
+     * base-omp/2nd_order_base_shll.c:43) */
+    if (t_end < 0) t_end = pb.dim == 1 ? 0.2f : (pb.order == 1 ? 0.1f : (pb.limiter == SHLL_LIM_MC ? 0.3f : 0.8f));
 
     nv = pb.dim == 1 ? 3 : 4;
     prim = (double *)malloc(sizeof(double) * (size_t)nv * (size_t)(pb.nx * pb.ny));
@@ -78,6 +89,7 @@ int main(int argc, char **argv)
     if (!strcmp(ic, "sod")) shll_ic_tube(pb.nx, 0.5, 10.0, 0.0, 1.0, 1.0, 0.0, 1.0, prim);
     else if (!strcmp(ic, "implosion")) shll_ic_implosion(pb.nx, pb.ny, prim);
     else if (!strcmp(ic, "four-shocks")) shll_ic_four_shocks(pb.nx, pb.ny, prim);
+    else if (!strcmp(ic, "four-contacts")) shll_ic_four_contacts(pb.nx, pb.ny, prim);
     else if (!strcmp(ic, "circle")) shll_ic_circle(pb.nx, pb.ny, 1.0, 1.0, 0.5, 0.5, 0.25, 10.0, 1.0, 1.0, 1.0, prim);
     else { fprintf(stderr, "unknown --ic %s\n", ic); free(prim); return 2; }
 
@@ -102,7 +114,7 @@ int main(int argc, char **argv)
         }
     }
     if (out && strcmp(out, "-") != 0) {
-        if (shll_write_results(ctx, out) != SHLL_OK) {
+        if ((full ? shll_write_results_full(ctx, out) : shll_write_results(ctx, out)) != SHLL_OK) {
             fprintf(stderr, "write: %s\n", shll_last_error());
             shll_free(ctx);
             return 1;

@@ -23,6 +23,8 @@ LIB_PATH = os.environ.get("SHLL_AMD_LIB", os.path.join(PKG, "lib", "libshll_amd.
 
 F64, REF32, C32, C32K = 0, 1, 2, 3
 BC_REFLECT, BC_OUTFLOW = 0, 1
+LIM_MINMOD, LIM_MC = 0, 1
+LIMITER_NAMES = {"minmod": LIM_MINMOD, "mc": LIM_MC}
 PREC_NAMES = {"f64": F64, "ref32": REF32, "c32": C32, "c32k": C32K}
 
 OK, EINVAL, ENOMEM, EDEVICE, ENONFINITE, ESTATE = 0, 1, 2, 3, 4, 5
@@ -37,7 +39,8 @@ class ShllError(RuntimeError):
 class Problem(ctypes.Structure):
     _fields_ = [("dim", ctypes.c_int32), ("order", ctypes.c_int32), ("bc", ctypes.c_int32),
                 ("precision", ctypes.c_int32), ("nx", ctypes.c_int64), ("ny", ctypes.c_int64),
-                ("lx", ctypes.c_double), ("ly", ctypes.c_double)]
+                ("lx", ctypes.c_double), ("ly", ctypes.c_double), ("limiter", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("theta", ctypes.c_double)]
 
 
 class Errors(ctypes.Structure):
@@ -72,7 +75,7 @@ EXPORTS = [
     "shll_batch_autotune", "shll_batch_get_launch", "shll_batch_check_state", "shll_batch_compare",
     "shll_batch_set_timing", "shll_batch_last_timing",
     "shll_batch_total_cell_updates", "shll_batch_reset", "shll_batch_free", "shll_ic_tube",
-    "shll_ic_implosion", "shll_ic_four_shocks", "shll_ic_circle", "shll_gen_tube", "shll_gen_domain",
+    "shll_ic_implosion", "shll_ic_four_shocks", "shll_ic_four_contacts", "shll_ic_circle", "shll_gen_tube", "shll_gen_domain",
     "shll_ic_domain", "shll_last_error", "shll_version",
 ]
 
@@ -131,6 +134,7 @@ def lib():
         "shll_ic_tube": (None, [i64] + [ctypes.c_double] * 7 + [dp]),
         "shll_ic_implosion": (None, [i64, i64, dp]),
         "shll_ic_four_shocks": (None, [i64, i64, dp]),
+        "shll_ic_four_contacts": (None, [i64, i64, dp]),
         "shll_ic_circle": (None, [i64, i64] + [ctypes.c_double] * 9 + [dp]),
         "shll_gen_tube": (None, [ctypes.c_uint64, i64, i64, i64, ctypes.POINTER(TubeSpec)]),
         "shll_gen_domain": (None, [ctypes.c_uint64, i64, i64, i64, ctypes.POINTER(DomainSpec)]),
@@ -155,13 +159,16 @@ def _dp(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
 
-def problem(dim, order, nx, ny=1, precision=F64, bc=None, lx=1.0, ly=1.0):
-    """The reference pairs first order with reflective walls and second order with outflow."""
+def problem(dim, order, nx, ny=1, precision=F64, bc=None, lx=1.0, ly=1.0, limiter=LIM_MINMOD, theta=1.25):
+    """The reference pairs first order with reflective walls and second order with outflow;
+    limiter="mc" with theta = 1.25 is base-omp/2nd_order_base_shll.c:44,317-325."""
     if isinstance(precision, str):
         precision = PREC_NAMES[precision]
+    if isinstance(limiter, str):
+        limiter = LIMITER_NAMES[limiter]
     if bc is None:
         bc = BC_REFLECT if order == 1 else BC_OUTFLOW
-    return Problem(dim, order, bc, precision, nx, ny if dim == 2 else 1, lx, ly)
+    return Problem(dim, order, bc, precision, nx, ny if dim == 2 else 1, lx, ly, limiter, 0, theta)
 
 
 def nvar(pb):
@@ -396,6 +403,12 @@ def ic_implosion(nx, ny):
 def ic_four_shocks(nx, ny):
     out = np.zeros((4, nx * ny))
     lib().shll_ic_four_shocks(nx, ny, _dp(out))
+    return out
+
+
+def ic_four_contacts(nx, ny):
+    out = np.zeros((4, nx * ny))
+    lib().shll_ic_four_contacts(nx, ny, _dp(out))
     return out
 
 

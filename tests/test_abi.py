@@ -46,6 +46,8 @@ def test_initial_data_matches_oracle(orc):
     np.testing.assert_array_equal(S.ic_implosion(100, 60), orc.ic_implosion(100, 60))
     np.testing.assert_array_equal(S.ic_four_shocks(64, 96), orc.ic_four_shocks(64, 96))
     np.testing.assert_array_equal(S.ic_circle(128, 96), orc.ic_circle(128, 96))
+    for nx, ny in ((64, 96), (1024, 1024), (33, 17)):
+        np.testing.assert_array_equal(S.ic_four_contacts(nx, ny), orc.ic_four_contacts(nx, ny))
 
 
 def test_batch_generators_are_counter_based_and_in_range():
@@ -69,7 +71,9 @@ def test_invalid_problems_are_rejected_before_any_device_call():
     prim = np.zeros((3, 8))
     h = ctypes.c_void_p()
     for pb in (S.problem(3, 1, 8), S.problem(1, 3, 8), S.problem(1, 1, 1), S.problem(1, 1, 8, precision=7),
-               S.Problem(1, 1, 0, 0, 8, 2, 1.0, 1.0), S.problem(2, 1, 8, 1), S.problem(1, 1, 8, lx=0.0)):
+               S.Problem(1, 1, 0, 0, 8, 2, 1.0, 1.0), S.problem(2, 1, 8, 1), S.problem(1, 1, 8, lx=0.0),
+               S.problem(1, 2, 8, limiter=2), S.problem(1, 1, 8, limiter=S.LIM_MC),
+               S.problem(1, 2, 8, limiter=S.LIM_MC, theta=0.5), S.problem(2, 2, 8, 8, limiter=S.LIM_MC, theta=2.5)):
         rc = S.lib().shll_init(ctypes.byref(pb), S._dp(prim), ctypes.byref(h))
         assert rc == S.EINVAL, (pb.dim, pb.order, pb.nx, pb.ny, rc)
         assert S.lib().shll_last_error()

@@ -22,7 +22,8 @@ def banner(*a):
 
 
 def opb(orc, pb):
-    return orc.problem(pb.dim, pb.order, pb.nx, pb.ny, bc=pb.bc, lx=pb.lx, ly=pb.ly)
+    return orc.problem(pb.dim, pb.order, pb.nx, pb.ny, bc=pb.bc, lx=pb.lx, ly=pb.ly, limiter=pb.limiter,
+                       theta=pb.theta)
 
 
 def gpu_run(pb, prim, nsteps, launch=None):
@@ -292,3 +293,77 @@ def test_full_results_file_holds_primitives_and_conserved(tmp_path, dim):
         np.testing.assert_array_equal(r[n].ravel(), short[n].ravel())
         np.testing.assert_allclose(r[n].ravel(), p[k], rtol=1e-6)
         np.testing.assert_allclose(r["u%d" % k].ravel(), c[k], rtol=1e-6, atol=1e-30)
+
+
+# ---------------------------------------------------------------------------
+# MC limiter (base-omp/2nd_order_base_shll.c:44,317-325,385-392)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("bc", [0, 1])
+@pytest.mark.parametrize("prec", [S.F64, S.REF32, S.C32, S.C32K])
+def test_1d_mc_limiter(orc, prec, bc):
+    n = 1500
+    banner("1d mc", prec, bc)
+    for theta, launch in ((1.25, (2, 2, 1, 0)), (1.25, (8, 5, 4, 20)), (2.0, (4, 7, 4, 0))):
+        pb = S.problem(1, 2, n, precision=prec, bc=bc, limiter=S.LIM_MC, theta=theta)
+        ic = icgen.random_1d(n, 60 + bc, block=1)
+        assert_same(pb, gpu_run(pb, ic, 3, launch), oracle_run(orc, pb, ic, 3), "mc rough %s" % (launch,))
+        ic = icgen.random_1d(n, 70 + bc, block=29)
+        assert_same(pb, gpu_run(pb, ic, 80, launch), oracle_run(orc, pb, ic, 80), "mc blocky %s" % (launch,))
+    pb = S.problem(1, 2, 1024, precision=prec, limiter=S.LIM_MC)
+    ic = S.ic_tube(1024)
+    assert_same(pb, gpu_run(pb, ic, 1639), oracle_run(orc, pb, ic, 1639), "mc sod")
+
+
+@pytest.mark.parametrize("prec", [S.F64, S.REF32, S.C32, S.C32K])
+def test_2d_mc_limiter(orc, prec):
+    banner("2d mc", prec)
+    pb = S.problem(2, 2, 70, 150, precision=prec, limiter=S.LIM_MC)
+    ic = icgen.random_2d(70, 150, 19, block=1)
+    assert_same(pb, gpu_run(pb, ic, 2), oracle_run(orc, pb, ic, 2), "2d mc rough")
+    ic = icgen.random_2d(70, 150, 23, block=11)
+    assert_same(pb, gpu_run(pb, ic, 25), oracle_run(orc, pb, ic, 25), "2d mc blocky")
+    pb = S.problem(2, 2, 128, 96, precision=prec, limiter=S.LIM_MC)
+    ic = S.ic_four_contacts(128, 96)
+    assert_same(pb, gpu_run(pb, ic, 60), oracle_run(orc, pb, ic, 60), "2d mc four contacts")
+
+
+@pytest.mark.parametrize("name", ["mc_2d_contacts", "mc_2d_rand", "mc_2d_blocks"])
+def test_2d_mc_reference_golden(name):
+    """REF32 with the MC limiter reproduces base-omp/2nd_order_base_shll.c at its own
+    1024^2 size bitwise (digests of every array after each recorded step)."""
+    import hashlib
+    import json
+    g = np.load(os.path.join(GOLD, name + ".npz"), allow_pickle=False)
+    case = json.load(open(os.path.join(GOLD, "manifest.json")))["cases"][name]
+    n = case["side"]
+    pb = S.problem(2, 2, n, n, precision=S.REF32, limiter=S.LIM_MC, theta=1.25)
+    spec = case["ic"]
+    ic = S.ic_four_contacts(n, n) if spec is None else icgen.random_2d(n, n, spec["seed"], spec["block"])
+    banner("2d mc golden", name)
+    with S.Solver(pb, ic) as s:
+        done = 0
+        for k in case["steps"]:
+            s.step(k - done)
+            done = k
+            p = s.primitives().astype(np.float32)
+            u = s.state_raw()
+            dig = [hashlib.sha256(np.ascontiguousarray(x).tobytes()).hexdigest() for x in list(p) + list(u)]
+            assert dig == list(g["dig_%d" % k][:8]), (name, k)
+
+
+@pytest.mark.parametrize("prec", [S.F64, S.C32K])
+def test_1d_mc_batch_equals_single_runs(orc, prec):
+    """A batch of MC tubes with different theta gives each tube's single-run bits."""
+    banner("1d mc batch", prec)
+    sizes = [300, 1111, 4000]
+    pbs = [S.problem(1, 2, n, precision=prec, limiter=S.LIM_MC, theta=th) for n, th in zip(sizes, (1.0, 1.25, 2.0))]
+    ics = [icgen.random_1d(n, 90 + i, block=17) for i, n in enumerate(sizes)]
+    steps = [40, 75, 60]
+    b = S.Batch(pbs, ics, steps)
+    b.set_launch(4, 8, 1, 16)
+    b.run()
+    for k, (pb, ic, ns) in enumerate(zip(pbs, ics, steps)):
+        p, c, _ = oracle_run(orc, pb, ic, ns)
+        np.testing.assert_array_equal(b.conserved(k), c, err_msg="tube %d" % k)
+        np.testing.assert_array_equal(b.primitives(k), p, err_msg="tube %d" % k)
+    b.close()

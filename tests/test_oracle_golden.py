@@ -18,34 +18,45 @@ def load(name):
     return np.load(os.path.join(GOLD, name + ".npz"), allow_pickle=False)
 
 
+def side_of(case):
+    return case.get("side", N2)
+
+
 def ic_for(orc, case):
     spec = case["ic"]
+    n = side_of(case)
     if spec is None:
         if case["scheme"] == "1d_o1":
-            return orc.ic_tube(256)
+            return orc.ic_tube(n)
         if case["scheme"] == "2d_o1":
-            return orc.ic_implosion(N2, N2)
-        return orc.ic_four_shocks(N2, N2)
+            return orc.ic_implosion(n, n)
+        if case["scheme"] == "2d_o2mc":
+            return orc.ic_four_contacts(n, n)
+        return orc.ic_four_shocks(n, n)
     k = spec["kind"]
     if k == "random_1d":
-        return icgen.random_1d(256, spec["seed"], spec["block"])
+        return icgen.random_1d(n, spec["seed"], spec["block"])
     if k == "random_2d":
-        return icgen.random_2d(N2, N2, spec["seed"], spec["block"])
+        return icgen.random_2d(n, n, spec["seed"], spec["block"])
     if k == "random_2d_yuni":
-        return icgen.random_2d(N2, N2, spec["seed"], spec["block"], y_uniform=True)
+        return icgen.random_2d(n, n, spec["seed"], spec["block"], y_uniform=True)
     if k == "sod_y_uniform":
-        return icgen.sod_2d_y_uniform(N2, N2)
+        return icgen.sod_2d_y_uniform(n, n)
     if k == "circle":
-        return icgen.circle_2d(N2, N2)
+        return icgen.circle_2d(n, n)
     raise ValueError(k)
 
 
-def problem_for(orc, scheme):
+def problem_for(orc, scheme, n=N2):
+    """The reference file of each scheme: base_shll.c, base_shll_2d.c, 2nd_order_base_shll.c,
+    base-omp/2nd_order_base_shll.c (MC limiter, alpha = 1.25)."""
     if scheme == "1d_o1":
-        return orc.problem(1, 1, 256)
+        return orc.problem(1, 1, n)
     if scheme == "2d_o1":
-        return orc.problem(2, 1, N2, N2)
-    return orc.problem(2, 2, N2, N2)
+        return orc.problem(2, 1, n, n)
+    if scheme == "2d_o2mc":
+        return orc.problem(2, 2, n, n, limiter=orc.LIM_MC, theta=1.25)
+    return orc.problem(2, 2, n, n)
 
 
 def digest(a):
@@ -58,7 +69,7 @@ CASES = sorted(MANIFEST["cases"].items())
 @pytest.mark.parametrize("name,case", CASES, ids=[c[0] for c in CASES])
 def test_ref32_matches_reference(orc, name, case):
     g = load(name)
-    pb = problem_for(orc, case["scheme"])
+    pb = problem_for(orc, case["scheme"], side_of(case))
     ic = ic_for(orc, case)
     for k in case["steps"]:
         if k == 0:
@@ -99,15 +110,17 @@ def test_reference_step_counts(orc):
         assert str(MANIFEST["cases"][name]["steps"][0]) in bytes(load(name)["stdout"]).decode()
 
 
-@pytest.mark.parametrize("name", ["o2_2d_sody", "o2_2d_yuni"])
+@pytest.mark.parametrize("name", ["o2_2d_sody", "o2_2d_yuni", "mc_2d_sody", "mc_2d_yuni"])
 def test_1d_second_order_is_the_2d_file_on_y_uniform_data(orc, name):
     """The derived 1D second-order scheme equals every x line of the reference's 2D
-    second-order solver run on y-uniform data with v = 0 (bitwise)."""
+    second-order solver run on y-uniform data with v = 0 (bitwise); with either limiter."""
     case = MANIFEST["cases"][name]
     g = load(name)
+    n = side_of(case)
     ic2 = ic_for(orc, case)
-    ic1 = ic2.reshape(4, N2, N2)[[0, 1, 3], :, 0]
-    pb = orc.problem(1, 2, N2)
+    ic1 = ic2.reshape(4, n, n)[[0, 1, 3], :, 0]
+    lim = orc.LIM_MC if case["scheme"] == "2d_o2mc" else orc.LIM_MINMOD
+    pb = orc.problem(1, 2, n, limiter=lim)
     for k in case["steps"]:
         if k == 0:
             st = orc.stage(pb, orc.REF32, ic1)
@@ -123,9 +136,11 @@ def test_initial_conditions_match_reference(orc):
     """The oracle's IC generators reproduce the reference's own step-0 arrays."""
     g = load("o1_1d_sod")
     np.testing.assert_array_equal(orc.ic_tube(256).astype(np.float32), g["p_0"])
-    for name, ic in (("o1_2d_implosion", orc.ic_implosion(N2, N2)), ("o2_2d_fourshocks", orc.ic_four_shocks(N2, N2))):
+    for name, ic in (("o1_2d_implosion", orc.ic_implosion(N2, N2)), ("o2_2d_fourshocks", orc.ic_four_shocks(N2, N2)),
+                     ("mc_2d_contacts", orc.ic_four_contacts(1024, 1024))):
         gg = load(name)
-        st = orc.stage(problem_for(orc, MANIFEST["cases"][name]["scheme"]), orc.REF32, ic)
+        case = MANIFEST["cases"][name]
+        st = orc.stage(problem_for(orc, case["scheme"], side_of(case)), orc.REF32, ic)
         assert [digest(st["prim"][v]) for v in range(4)] == list(gg["dig_0"][:4])
 
 

@@ -250,3 +250,77 @@ def test_kahan_and_twosum_updates_agree(orc):
     _, c3 = orc.run(pb, orc.C32K, ic, 400)
     assert np.abs(c3 - c2).max() <= 1e-6 * np.abs(c2).max()
     assert not np.array_equal(c2, c3)  # they are different schemes
+
+
+def textbook_mc(dl, dr, theta):
+    """Generalised MC limiter as a three-way minmod (van Leer 1977; LeVeque, Finite Volume
+    Methods for Hyperbolic Problems, 6.12): minmod(theta*dl, (dl+dr)/2, theta*dr)."""
+    c = [theta * dl, 0.5 * (dl + dr), theta * dr]
+    if all(x > 0 for x in c):
+        return min(c)
+    if all(x < 0 for x in c):
+        return max(c)
+    return 0.0
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_mc_slope_is_the_textbook_limiter(orc, mode):
+    """On dyadic data (every operation exact) the MC slope of each mode equals the
+    three-way minmod form for theta in {1.25, 1.5, 2}, and reduces to minmod at theta = 1."""
+    vals = [-3.0, -1.5, -1.0, -0.25, 0.0, 0.5, 1.0, 1.75, 4.0]
+    for theta in (1.0, 1.25, 1.5, 2.0):
+        for ql in vals:
+            for qc in vals:
+                for qr in vals:
+                    dl, dr = qc - ql, qr - qc
+                    got = orc.slope(mode, orc.LIM_MC, theta, ql, qc, qr)
+                    assert got == textbook_mc(dl, dr, theta), (mode, theta, ql, qc, qr, got)
+                    if theta == 1.0:
+                        assert got == orc.slope(mode, orc.LIM_MINMOD, theta, ql, qc, qr)
+    # an MC slope exceeds the minmod slope where the data are smooth but uneven
+    assert orc.slope(mode, orc.LIM_MC, 1.25, 0.0, 1.0, 3.0) == 1.25
+    assert orc.slope(mode, orc.LIM_MINMOD, 1.25, 0.0, 1.0, 3.0) == 1.0
+
+
+@pytest.mark.parametrize("mode", [2, 3])
+def test_c32_mc_accuracy(orc, mode):
+    """With the MC limiter the compensated schemes stay within L1(rho) <= 1e-6 of FP64."""
+    pb = orc.problem(1, 2, 1024, limiter=orc.LIM_MC)
+    ic = orc.ic_tube(1024)
+    pf, _ = orc.run(pb, orc.F64, ic, 1639)
+    pc, _ = orc.run(pb, mode, ic, 1639)
+    assert np.abs(pc[0] - pf[0]).sum() / 1024 <= 1e-6
+
+
+def test_mc_is_sharper_than_minmod(orc):
+    """Second-order Sod at t = 0.2: the MC limiter's L1 error against the exact solution is
+    below minmod's (it clips less at smooth extrema), and both are below first order's."""
+    n = 1024
+    errs = {}
+    for name, order, lim in (("o1", 1, 0), ("minmod", 2, 0), ("mc", 2, 1)):
+        pb = orc.problem(1, order, n, bc=1, limiter=lim)
+        steps = orc.steps_to_time(pb, 0.2)
+        t = steps * orc.constants(pb, orc.F64)["dt"]
+        p, _ = orc.run(pb, orc.F64, orc.ic_tube(n), steps)
+        ex, _, _ = exact_riemann_rho((np.arange(n) + 0.5) / n, t, 10.0, 10.0, 1.0, 1.0)
+        errs[name] = np.abs(p[0] - ex).sum() / n
+    assert errs["mc"] < errs["minmod"] < errs["o1"], errs
+
+
+@pytest.mark.parametrize("mode", [0, 2])
+def test_mc_mirror_symmetry_and_conservation(orc, mode):
+    """MC limiter, reflective walls: the mirrored problem gives the mirrored solution and
+    mass and energy are conserved."""
+    n = 1024
+    pb = orc.problem(1, 2, n, bc=0, limiter=orc.LIM_MC)
+    ic = orc.ic_tube(n, 0.5, 3.0, 0.25, 1.25, 1.0, -0.5, 1.0)
+    icm = ic[:, ::-1].copy()
+    icm[1] *= -1
+    c0 = orc.stage(pb, mode, ic)["cons"].sum(axis=1)
+    p, c = orc.run(pb, mode, ic, 800)
+    pm, _ = orc.run(pb, mode, icm, 800)
+    tol = 1e-10 if mode == orc.F64 else 2e-5
+    np.testing.assert_allclose(pm[0][::-1], p[0], rtol=tol)
+    np.testing.assert_allclose(-pm[1][::-1], p[1], atol=tol * np.abs(p[1]).max())
+    c1 = c.sum(axis=1)
+    assert abs(c1[0] - c0[0]) <= 1e-12 * c0[0] and abs(c1[2] - c0[2]) <= 1e-12 * c0[2]
