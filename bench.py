@@ -212,7 +212,8 @@ def extra_single(S):
         dt = time.perf_counter() - t0
         sol[name] = s
         out["c2_2d_o1_%s_4096sq" % name] = {"steps": 2000, "seconds": dt,
-                                           "cell_updates_per_s": n2 * n2 * 2000 / dt}
+                                           "cell_updates_per_s": n2 * n2 * 2000 / dt,
+                                           "launch": list(s.get_launch())}
     for name in ("c32", "c32k"):
         out["c2_2d_o1_%s_4096sq" % name]["error_vs_f64"] = sol[name].compare(sol["f64"])
     for s in sol.values():
@@ -236,9 +237,11 @@ def extra_single(S):
         b.run()
         dt = time.perf_counter() - t0
         errs = [b.compare(k, ref)["l1_rho"] / (pbs[k].lx * pbs[k].ly) for k in range(64)]
+        launch = list(b.get_launch())
         nonfinite = bad64 + int((b.check_state() > 0).sum())
         b.close()
         out["c4_2d_o1_%s_64domains" % name] = {"steps": 1000, "cells": cells, "seconds": dt,
+                                               "launch": launch,
                                                "cell_updates_per_s": cells * 1000 / dt,
                                                "f64_batch_cell_updates_per_s": cells * 1000 / dt64,
                                                "l1_rho_vs_f64_per_unit_area_max": max(errs),

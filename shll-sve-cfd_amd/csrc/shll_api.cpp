@@ -22,7 +22,9 @@
 namespace shll {
 hipError_t launch_step1d(int prec, int order, int lim, int C, int wpb, int n_items, hipStream_t s, const StepArgs& a);
 bool step1d_supported(int C, int wpb);
-hipError_t launch_step2d(int prec, int order, int lim, int wpb, int n_items, hipStream_t s, const StepArgs& a);
+hipError_t launch_step2d(int prec, int order, int lim, int k, int wpb, int n_items, hipStream_t s,
+                         const StepArgs& a);
+bool step2d_supported(int order, int k);
 hipError_t launch_init(int prec, int dim, int nblocks, int n_inst, hipStream_t st, const Inst* insts,
                        const double* prim, void* o0, void* o1, int64_t plane);
 hipError_t launch_output(int prec, int dim, int nblocks, int n_inst, hipStream_t st, const Inst* insts,
@@ -156,11 +158,15 @@ static int nwords(int prec) { return prec == SHLL_C32 || prec == SHLL_C32K ? 2 :
 // ---------------------------------------------------------------------------
 struct Launch {
     int C = 2;          // cells per lane (1D)
-    int k = 8;          // steps per launch (1D)
+    int k = 8;          // steps per launch
     int wpb = 4;        // waves per block
     int graph = 128;    // steps per graph (0: no graph)
-    int rows = 16;      // rows per 2D item
+    int rows = 16;      // owned rows per 2D item
 };
+
+// owned rows per 2D item for k steps per launch: an item recomputes k halo rows on each
+// side, so longer items waste less; 16 rows at one step per launch
+static int rows2d(int k) { return k <= 1 ? 16 : (k == 2 ? 32 : 64); }
 
 // 1D tiling of one instance of nx cells: segments of 64*C cells, halo order*k.  The
 // segment that holds a wall also holds that wall's ghost cell (-1 or nx), so the
@@ -192,24 +198,27 @@ static bool tile1d(int32_t inst, int64_t nx, int order, int C, int k, std::vecto
     return true;
 }
 
-static void tile2d(int32_t inst, int64_t nx, int64_t ny, int order, int rows, std::vector<Item2>& out) {
+// 2D tiling: strips of 64 columns (one wavefront) whose outer H = order*k columns on a
+// cut side are recomputed halo, times runs of `rows` owned rows (H halo rows each side).
+static void tile2d(int32_t inst, int64_t nx, int64_t ny, int order, int k, int rows, std::vector<Item2>& out) {
     struct Strip { int32_t j0, jx0, jw; };
     std::vector<Strip> strips;
+    const int64_t H = (int64_t)order * k;
     {
-        int64_t jw = std::min<int64_t>(ny, 64 - order);
+        int64_t jw = std::min<int64_t>(ny, 64 - H);
         strips.push_back(Strip{0, 0, (int32_t)jw});
         int64_t jx0 = jw;
         while (jx0 < ny) {
-            int64_t w = std::min<int64_t>(ny - jx0, 64 - 2 * order);
-            strips.push_back(Strip{(int32_t)(jx0 - order), (int32_t)jx0, (int32_t)w});
+            int64_t w = std::min<int64_t>(ny - jx0, 64 - 2 * H);
+            strips.push_back(Strip{(int32_t)(jx0 - H), (int32_t)jx0, (int32_t)w});
             jx0 += w;
         }
     }
     for (int64_t ix0 = 0; ix0 < nx; ix0 += rows) {
         const int64_t iw = std::min<int64_t>(rows, nx - ix0);
         for (const Strip& s : strips)
-            out.push_back(Item2{inst, s.j0, s.jx0, s.jw, (int32_t)(ix0 - order), (int32_t)ix0, (int32_t)iw,
-                                (int32_t)(iw + 2 * order)});
+            out.push_back(Item2{inst, s.j0, s.jx0, s.jw, (int32_t)(ix0 - H), (int32_t)ix0, (int32_t)iw,
+                                (int32_t)(iw + 2 * H)});
     }
 }
 
@@ -308,7 +317,7 @@ static void engine_free(Engine* E) {
 static int radius(const Engine* E) { return E->order; }
 
 // steps this launch shape advances per launch
-static int launch_k(const Engine* E) { return E->dim == 1 ? E->L.k : 1; }
+static int launch_k(const Engine* E) { return E->L.k; }
 
 static int64_t active_items(const Engine* E, int64_t j);
 
@@ -326,7 +335,7 @@ static shll_status engine_build_items(Engine* E) {
                 return fail(SHLL_EINVAL, "launch shape C=%d k=%d leaves no owned cells (order %d)", E->L.C, E->L.k,
                             E->order);
         } else {
-            tile2d((int32_t)i, in.nx, in.ny, E->order, E->L.rows, it2);
+            tile2d((int32_t)i, in.nx, in.ny, E->order, E->L.k, E->L.rows, it2);
         }
     }
     E->inst_first_item[E->order_idx.size()] = E->dim == 1 ? (int64_t)it1.size() : (int64_t)it2.size();
@@ -400,7 +409,7 @@ static StepArgs make_args(const Engine* E, int64_t j, int parity, int64_t n_item
 
 static hipError_t launch_one(const Engine* E, const StepArgs& a, int64_t n_items) {
     if (E->dim == 1) return launch_step1d(E->prec, E->order, E->lim, E->L.C, E->L.wpb, (int)n_items, E->stream, a);
-    return launch_step2d(E->prec, E->order, E->lim, E->L.wpb, (int)n_items, E->stream, a);
+    return launch_step2d(E->prec, E->order, E->lim, E->L.k, E->L.wpb, (int)n_items, E->stream, a);
 }
 
 // Single instance: every launch has the same items; ksteps varies only on the last one.
@@ -707,12 +716,15 @@ static shll_status engine_create_stream(Engine* E) {
 
 static void default_launch(Engine* E, int64_t max_cells) {
     if (E->dim == 2) {
-        // 16-row items: a 4096^2 domain in 64-row items leaves ~4 wavefronts per SIMD,
-        // too few to cover the row loads; 16 rows measured 10-30% faster
+        // First order: several steps per launch (row-pipelined stages); measured on a
+        // 4096^2 domain and a 64-domain batch, the FP32 forms peak at 4 steps per launch,
+        // FP64 (twice the registers per stage) at 2.  Second order: one step per launch
+        // in 16-row items (a 4096^2 domain in 64-row items leaves ~4 wavefronts per SIMD,
+        // too few to cover the row loads).
         E->L.C = 1;
-        E->L.k = 1;
+        E->L.k = E->order == 1 ? (E->prec == PREC_C32 || E->prec == PREC_C32K ? 4 : 2) : 1;
         E->L.wpb = 4;
-        E->L.rows = 16;
+        E->L.rows = rows2d(E->L.k);
         E->L.graph = 64;
         return;
     }
@@ -999,11 +1011,14 @@ static shll_status set_launch(Engine* E, int32_t C, int32_t k, int32_t wpb, int3
         return fail(SHLL_EINVAL, "unsupported launch shape C=%d waves/block=%d", C, wpb);
     if (E->dim == 2 && wpb != 1 && wpb != 4) return fail(SHLL_EINVAL, "waves/block must be 1 or 4");
     if (k < 1 || graph < 0) return fail(SHLL_EINVAL, "steps per launch must be >= 1, graph steps >= 0");
+    if (E->dim == 2 && !step2d_supported(E->order, k))
+        return fail(SHLL_EINVAL, "2D steps per launch must be 1 (second order) or 1, 2, 3, 4, 8 (first order); got %d", k);
     if (E->batch && E->launches_done != 0)
         return fail(SHLL_ESTATE, "a batch's launch shape can only change before it advances (reset first)");
     Launch old = E->L;
     E->L.C = E->dim == 1 ? C : 1;
-    E->L.k = E->dim == 1 ? k : 1;
+    E->L.k = k;
+    if (E->dim == 2) E->L.rows = rows2d(k);
     E->L.wpb = wpb;
     E->L.graph = graph;
     E->items_valid = false;
@@ -1114,11 +1129,15 @@ static shll_status engine_autotune(Engine* E, int32_t trial_steps, int64_t shape
                     cands.push_back(L);
                 }
     } else {
-        for (int w : {1, 4}) {
-            Launch L = start;
-            L.wpb = w;
-            cands.push_back(L);
-        }
+        for (int k : {1, 2, 3, 4, 8})
+            for (int w : {1, 4}) {
+                if (!step2d_supported(E->order, k)) continue;
+                Launch L = start;
+                L.k = k;
+                L.wpb = w;
+                L.graph = std::max(64, 2 * k);
+                cands.push_back(L);
+            }
     }
     double best = 1e300;
     Launch bestL = start;

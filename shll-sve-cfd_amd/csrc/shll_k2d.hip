@@ -257,6 +257,281 @@ __global__ __launch_bounds__(64 * WPB) void k_step2d(StepArgs a) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// First order, K steps per launch: a pipeline of K stages along the rows.
+//
+// Stage s takes the rows of its input level one at a time (stage 0 from HBM, stage s
+// from stage s-1) and holds the last row it took.  When row t arrives, the x face
+// between rows t-1 and t is known, so the held row t-1 is advanced one step and
+// passed on; row t is held in its place.  After the last row a stage drains its held
+// row with the wall face.  Each stage keeps only one row (state, x split fluxes and
+// the row's face and y terms), so K steps cost K times the arithmetic of one step but
+// one read and one write of the state.  The row next to a cut (an item edge that is
+// not a wall) gets a made-up face and a strip's edge lanes get their own values from
+// the DPP shifts, so each stage spoils one more row and column next to every cut: the
+// items' halos are K rows and K columns wide and only owned cells are stored.  Stages
+// at or past the instance's remaining step count pass rows through.
+// ---------------------------------------------------------------------------
+template <int PREC>
+struct Held2 {
+    using RT = typename Real2<PREC>::T;
+    RT u[4], l[4];    // state (C32: hi, lo)
+    RT fp[4];         // x split flux F+ (the next row's lower face)
+    RT d[4];          // F64/REF32: F+ - F- of the row
+    RT tl[4];         // C32: lam_x * G of the lower x face; F64/REF32: L (F+ of the row below)
+    RT yb[4], yt[4];  // C32: lam_y * G of the two y faces (C32K: yb = their difference);
+                      // F64/REF32: yb = (hp - hm) + T - B
+};
+
+// A lane's y walls, as per-lane flags (lane-varying, so the wall choices are selects
+// rather than branches): lo / hi = the lane is column 0 / ny-1, *_refl = and reflective
+struct LaneWalls {
+    bool lo, hi, lo_refl, hi_refl, refl;
+};
+
+enum { ST_TAKE_FIRST = 0, ST_TAKE = 1, ST_DRAIN = 2 };
+
+// One stage, one row.  io: the incoming row (ST_TAKE_FIRST / ST_TAKE); on return the
+// outgoing row (ST_TAKE / ST_DRAIN).  act: the stage advances (else it only delays).
+template <int PREC, bool YW>
+__device__ __forceinline__ void stage2(Held2<PREC>& h, typename Real2<PREC>::T (&iou)[4],
+                                       typename Real2<PREC>::T (&iol)[4], int mode, bool act, const Inst& in,
+                                       const LaneWalls& w, typename Real2<PREC>::T lamx,
+                                       typename Real2<PREC>::T lamy) {
+    using RT = typename Real2<PREC>::T;
+    constexpr bool CMP = PREC == PREC_C32 || PREC == PREC_C32K;
+    const bool refl = w.refl;
+    if (!act) {
+        // pass-through with the stage's one-row delay
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const RT u = h.u[v], l = h.l[v];
+            if (mode != ST_DRAIN) { h.u[v] = iou[v]; h.l[v] = iol[v]; }
+            iou[v] = u;
+            iol[v] = l;
+        }
+        return;
+    }
+    RT xp[4], xm[4], yp[4], ym[4], face[4], nu[4], nl[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        nu[v] = iou[v];  // the incoming row (iou is overwritten by the outgoing one)
+        nl[v] = iol[v];
+    }
+    // the x fluxes of the row that meets the held row's upper face: the incoming row, or
+    // at a drain the held row itself (its F- for the outflow wall)
+    fluxes2<PREC, RT>(mode == ST_DRAIN ? h.u : nu, in, xp, xm, yp, ym);
+    if (mode != ST_TAKE_FIRST) {
+        // advance the held row
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const RT sx = (RT)sgn_x(v);
+            const RT fpc = h.fp[v];
+            if constexpr (!CMP) {
+                const RT Rv = mode == ST_DRAIN ? (refl ? sx * fpc : xm[v]) : xm[v];
+                RT uu = h.u[v] - lamx * (h.d[v] + Rv - h.tl[v]);
+                uu = uu - lamy * h.yb[v];
+                iou[v] = uu;
+            } else {
+                // the face between the held and the incoming row (kept: the incoming row's lower face)
+                const RT Gr = mode == ST_DRAIN ? (refl ? fpc + sx * fpc : fpc + xm[v]) : fpc + xm[v];
+                const RT tl = h.tl[v], tr = lamx * Gr;
+                face[v] = tr;
+                RT aa = h.u[v], bb = h.l[v], e;
+                if constexpr (PREC == PREC_C32K) {
+                    const RT y = ((tl - tr) + h.yb[v]) + bb;
+                    const RT s2 = aa + y;
+                    e = y - (s2 - aa);
+                    aa = s2;
+                } else {
+                    two_sum(aa, tl, aa, e);
+                    bb = bb + e;
+                    two_sum(aa, -tr, aa, e);
+                    bb = bb + e;
+                    two_sum(aa, h.yb[v], aa, e);
+                    bb = bb + e;
+                    two_sum(aa, -h.yt[v], aa, e);
+                    bb = bb + e;
+                    fast_two_sum(aa, bb, aa, e);
+                }
+                iou[v] = aa;
+                iol[v] = e;
+            }
+        }
+    }
+    if (mode == ST_DRAIN) return;
+    // hold the incoming row: its lower x face (the wall rule on a stage's first row), y terms
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const RT sx = (RT)sgn_x(v), sy = (RT)sgn_y(v);
+        const RT fpc = xp[v], fmc = xm[v], hpc = yp[v], hmc = ym[v];
+        const RT hpl = from_left(hpc), hmr = from_right(hmc);
+        if constexpr (!CMP) {
+            h.tl[v] = mode == ST_TAKE_FIRST ? (refl ? sx * fmc : fpc) : h.fp[v];
+            const RT Bv = YW ? (w.lo_refl ? sy * hmc : (w.lo ? hpc : hpl)) : hpl;
+            const RT Tv = YW ? (w.hi_refl ? sy * hpc : (w.hi ? hmc : hmr)) : hmr;
+            h.yb[v] = hpc - hmc + Tv - Bv;
+            h.d[v] = fpc - fmc;
+        } else {
+            h.tl[v] = mode == ST_TAKE_FIRST ? lamx * (refl ? fmc + sx * fmc : fpc + fmc) : face[v];
+            // wall faces: (mirrored or own) flux + own flux; the sum commutes exactly
+            const RT Gb = (YW ? (w.lo_refl ? sy * hmc : (w.lo ? hpc : hpl)) : hpl) + hmc;
+            const RT tb = lamy * Gb;
+            RT tt = from_right(tb);
+            if constexpr (YW) tt = w.hi ? lamy * (hpc + (w.hi_refl ? sy * hpc : hmc)) : tt;
+            if constexpr (PREC == PREC_C32K) {
+                h.yb[v] = tb - tt;
+            } else {
+                h.yb[v] = tb;
+                h.yt[v] = tt;
+            }
+        }
+        h.u[v] = nu[v];
+        h.l[v] = nl[v];
+        h.fp[v] = fpc;
+    }
+}
+
+template <int PREC, int K, bool YW>
+__device__ __forceinline__ void pipeline2(const StepArgs& a, const Item2& it, const Inst& in, int lane, int ns) {
+    using RT = typename Real2<PREC>::T;
+    constexpr bool CMP = PREC == PREC_C32 || PREC == PREC_C32K;
+    const int nx = in.nx, ny = in.ny;
+    const int64_t P = a.plane;
+    const int j = it.j0 + lane;
+    const bool jok = j >= 0 && j < ny;
+    const bool jown = j >= it.jx0 && j < it.jx0 + it.jw;
+    const int jc = jok ? j : (j < 0 ? 0 : ny - 1);
+    const RT* __restrict__ i0p = (const RT*)a.in0 + in.off;
+    const RT* __restrict__ i1p = CMP ? (const RT*)a.in1 + in.off : nullptr;
+    RT* __restrict__ o0p = (RT*)a.out0 + in.off;
+    RT* __restrict__ o1p = CMP ? (RT*)a.out1 + in.off : nullptr;
+    const RT lamx = PREC == PREC_F64 ? (RT)in.k.d_lamx : (PREC == PREC_REF32 ? (RT)in.k.f_lamx : (RT)in.k.c_lamx);
+    const RT lamy = PREC == PREC_F64 ? (RT)in.k.d_lamy : (PREC == PREC_REF32 ? (RT)in.k.f_lamy : (RT)in.k.c_lamy);
+    const bool refl = in.bc == BC_REFLECT;
+    const LaneWalls w{j == 0, j == ny - 1, j == 0 && refl, j == ny - 1 && refl, refl};
+
+    // rows taken by every stage: [r0, r1), then a drain
+    const int r0 = it.ix0 - K < 0 ? 0 : it.ix0 - K;
+    const int r1 = it.ix0 + it.iw + K > nx ? nx : it.ix0 + it.iw + K;
+    Held2<PREC> h[K];
+#pragma unroll
+    for (int s = 0; s < K; ++s)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+            h[s].u[v] = h[s].l[v] = h[s].fp[v] = h[s].d[v] = h[s].tl[v] = h[s].yb[v] = h[s].yt[v] = (RT)0;
+
+    constexpr int D = PF_DEPTH;
+    RT nu[D][4], nl[D][4];
+    // jc is clamped into the domain: every lane loads, lanes off the domain keep a benign state
+    auto load_row = [&](int t, RT (&U)[4], RT (&Ul)[4]) {
+        if (t < r1) {
+            const int64_t o = (int64_t)t * ny + jc;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const RT x = i0p[v * P + o];
+                U[v] = jok ? x : (RT)(v == 0 ? 1.0 : (v == 3 ? 2.5 : 0.0));
+                if constexpr (CMP) {
+                    const RT y = i1p[v * P + o];
+                    Ul[v] = jok ? y : (RT)0;
+                } else {
+                    Ul[v] = (RT)0;
+                }
+            }
+        }
+    };
+#pragma unroll
+    for (int d = 0; d < D; ++d) load_row(r0 + d, nu[d], nl[d]);
+    // Stage s is at row t - s in iteration t: it takes rows [r0, r1) (from stage s-1,
+    // which gives up its held row when it takes the next one or drains) and drains at r1.
+    // Stage K-1 gives row t - K.
+    const int tend = r1 + K;
+    for (int t = r0; t < tend; ++t) {
+        RT U[4], Ul[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            U[v] = nu[0][v];
+            Ul[v] = nl[0][v];
+        }
+#pragma unroll
+        for (int d = 0; d + 1 < D; ++d)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                nu[d][v] = nu[d + 1][v];
+                nl[d][v] = nl[d + 1][v];
+            }
+        load_row(t + D, nu[D - 1], nl[D - 1]);
+        // live: U holds a row at the current stage's input level
+        bool live = true;
+        const bool steady = ns == K && t - (K - 1) > r0 && t < r1;
+        if (steady) {
+            // every stage takes an inner row and advances: one straight-line body
+#pragma unroll
+            for (int s = 0; s < K; ++s) stage2<PREC, YW>(h[s], U, Ul, ST_TAKE, true, in, w, lamx, lamy);
+        } else {
+#pragma unroll
+            for (int s = 0; s < K; ++s) {
+                const int ts = t - s;
+                if (ts < r0 || ts > r1) {
+                    live = false;
+                } else if (ts == r1) {
+                    stage2<PREC, YW>(h[s], U, Ul, ST_DRAIN, s < ns, in, w, lamx, lamy);
+                    live = true;
+                } else if (live) {
+                    const int mode = ts == r0 ? ST_TAKE_FIRST : ST_TAKE;
+                    stage2<PREC, YW>(h[s], U, Ul, mode, s < ns, in, w, lamx, lamy);
+                    live = mode == ST_TAKE;
+                }
+            }
+        }
+        const int c = t - K;
+        if (live && jown && c >= it.ix0 && c < it.ix0 + it.iw) {
+            const int64_t o = (int64_t)c * ny + jc;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                o0p[v * P + o] = U[v];
+                if constexpr (CMP) o1p[v * P + o] = Ul[v];
+            }
+        }
+    }
+}
+
+template <int PREC, int K, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_step2d_tb(StepArgs a) {
+    const int wave = (int)blockIdx.x * WPB + uniform((int)threadIdx.x >> 6);
+    const LaunchView lv = launch_view(a);
+    if (wave >= lv.n_items) return;
+    const int lane = (int)threadIdx.x & 63;
+    const Item2 it = ((const Item2*)a.items)[wave];
+    const Inst in = a.inst[it.inst];
+    const int64_t left = in.nsteps - lv.step_base;
+    if (left <= 0) return;
+    const int kk = a.ksteps < K ? a.ksteps : K;
+    const int ns = left < (int64_t)kk ? (int)left : kk;  // stages that advance
+    // strips away from both y walls take the copy without wall choices
+    const bool yw = it.j0 <= 0 || it.j0 + 64 >= in.ny;
+    if (yw) pipeline2<PREC, K, true>(a, it, in, lane, ns);
+    else pipeline2<PREC, K, false>(a, it, in, lane, ns);
+}
+
+template <int PREC, int WPB>
+static hipError_t launch2_tb_w(int k, int nblocks, hipStream_t s, const StepArgs& a) {
+    const dim3 grid(nblocks), block(64 * WPB);
+    switch (k) {
+    case 2: hipLaunchKernelGGL((k_step2d_tb<PREC, 2, WPB>), grid, block, 0, s, a); break;
+    case 3: hipLaunchKernelGGL((k_step2d_tb<PREC, 3, WPB>), grid, block, 0, s, a); break;
+    case 4: hipLaunchKernelGGL((k_step2d_tb<PREC, 4, WPB>), grid, block, 0, s, a); break;
+    case 8: hipLaunchKernelGGL((k_step2d_tb<PREC, 8, WPB>), grid, block, 0, s, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <int PREC>
+static hipError_t launch2_tb(int k, int wpb, int nblocks, hipStream_t s, const StepArgs& a) {
+    return wpb == 1 ? launch2_tb_w<PREC, 1>(k, nblocks, s, a) : launch2_tb_w<PREC, 4>(k, nblocks, s, a);
+}
+
 template <int PREC, int ORDER, int LIM>
 static hipError_t launch2_o(int wpb, int nblocks, hipStream_t s, const StepArgs& a) {
     if (wpb == 1) hipLaunchKernelGGL((k_step2d<PREC, ORDER, 1, LIM>), dim3(nblocks), dim3(64), 0, s, a);
@@ -274,9 +549,22 @@ static hipError_t launch2_p(int order, int lim, int wpb, int nblocks, hipStream_
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_step2d(int prec, int order, int lim, int wpb, int n_items, hipStream_t s, const StepArgs& a) {
+bool step2d_supported(int order, int k) { return k == 1 || (order == 1 && (k == 2 || k == 3 || k == 4 || k == 8)); }
+
+hipError_t launch_step2d(int prec, int order, int lim, int k, int wpb, int n_items, hipStream_t s,
+                         const StepArgs& a) {
     const int nblocks = (n_items + wpb - 1) / wpb;
     if (nblocks <= 0) return hipSuccess;
+    if (k != 1) {
+        if (order != 1 || lim != LIM_MINMOD) return hipErrorInvalidValue;
+        switch (prec) {
+        case PREC_F64: return launch2_tb<PREC_F64>(k, wpb, nblocks, s, a);
+        case PREC_REF32: return launch2_tb<PREC_REF32>(k, wpb, nblocks, s, a);
+        case PREC_C32: return launch2_tb<PREC_C32>(k, wpb, nblocks, s, a);
+        case PREC_C32K: return launch2_tb<PREC_C32K>(k, wpb, nblocks, s, a);
+        default: return hipErrorInvalidValue;
+        }
+    }
     switch (prec) {
     case PREC_F64: return launch2_p<PREC_F64>(order, lim, wpb, nblocks, s, a);
     case PREC_REF32: return launch2_p<PREC_REF32>(order, lim, wpb, nblocks, s, a);

@@ -367,3 +367,49 @@ def test_1d_mc_batch_equals_single_runs(orc, prec):
         np.testing.assert_array_equal(b.conserved(k), c, err_msg="tube %d" % k)
         np.testing.assert_array_equal(b.primitives(k), p, err_msg="tube %d" % k)
     b.close()
+
+
+# ---------------------------------------------------------------------------
+# 2D first order, several steps per launch (row-pipelined stages)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("k", [2, 4, 8])
+@pytest.mark.parametrize("prec", [S.F64, S.REF32, S.C32, S.C32K])
+def test_2d_steps_per_launch(orc, prec, k):
+    """k steps per launch give the oracle's bits: both walls, several row items and column
+    strips, a last launch with fewer than k steps, graphs and direct launches."""
+    for bc, (nx, ny), wpb, graph in ((0, (150, 130), 1, 0), (1, (203, 70), 4, 4 * k), (0, (9, 7), 4, 0)):
+        pb = S.problem(2, 1, nx, ny, precision=prec, bc=bc)
+        ic = icgen.random_2d(nx, ny, 31 + bc, block=9)
+        steps = 3 * k + 1
+        banner("2d k", prec, k, bc, nx, ny)
+        assert_same(pb, gpu_run(pb, ic, steps, (1, k, wpb, graph)), oracle_run(orc, pb, ic, steps),
+                    "2d k=%d %dx%d" % (k, nx, ny))
+    pb = S.problem(2, 1, 96, 96, precision=prec)
+    ic = S.ic_circle(96, 96)
+    assert_same(pb, gpu_run(pb, ic, 60, (1, k, 4, 0)), oracle_run(orc, pb, ic, 60), "2d k circle")
+
+
+def test_2d_steps_per_launch_rejected_in_second_order():
+    pb = S.problem(2, 2, 64, 64)
+    with S.Solver(pb, S.ic_four_shocks(64, 64)) as s:
+        with pytest.raises(S.ShllError) as e:
+            s.set_launch(1, 4, 4, 0)
+        assert e.value.code == S.EINVAL
+        s.step(2)
+
+
+@pytest.mark.parametrize("k", [2, 4, 8])
+@pytest.mark.parametrize("prec", [S.F64, S.C32K])
+def test_2d_batch_steps_per_launch(orc, prec, k):
+    """A batch of uneven domains at k steps per launch: each domain equals its own run,
+    including domains that stop in the middle of a launch."""
+    pbs, prims = S.domain_batch(57, 5, smin=20, smax=140, precision=prec)
+    steps = [4 * k + 3, 2 * k, 11, 1, 5 * k]
+    b = S.Batch(pbs, prims, steps)
+    b.set_launch(1, k, 4, 2 * k)
+    banner("2d batch k", prec, k)
+    b.run()
+    for i in range(len(pbs)):
+        ref = oracle_run(orc, pbs[i], prims[i], steps[i])
+        np.testing.assert_array_equal(b.conserved(i), ref[1], err_msg="domain %d" % i)
+    b.close()
