@@ -19,7 +19,14 @@ order 65536 cells and 2D first order 4096^2 in FP64 and both FP32-compensated fo
 cpu_baseline: the oracle's reference-precision port of the reference C loop (unfused
 passes, gcc -O3) on one host core, on a stratified sample of the same tubes.
 
+cpu_baseline_omp: the same port with one OpenMP worksharing loop per pass (the base-omp
+pattern) on up to 16 host threads.
+
+--workload domains times the 64 uneven 2D first-order domains instead (same JSON line,
+2D roofline), for profiling the 2D kernel.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extra]
+                       [--workload tubes|domains] [--form c32k|c32] [--launch C,k,waves,graph]
 """
 import argparse
 import json
@@ -54,13 +61,13 @@ def bytes_per_update(dim, prec):
     return 2 * nv * s
 
 
-def traffic_per_update(shape, form="c32k"):
+def traffic_per_update(shape, form="c32k", summary=None):
     """Calibrated HBM bytes per cell-update of the batch step kernel, from the committed
     rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE scaled by a 1 GiB copy), when the
     summary was taken with the same launch shape (cells/lane, steps/launch, waves/block);
     else None."""
     import re
-    path = os.path.join(ROOT, "profiles", "k_step1d_%s_o2_tubes4096_hbm_traffic.md" % form)
+    path = os.path.join(ROOT, "profiles", summary or "k_step1d_%s_o2_tubes4096_hbm_traffic.md" % form)
     if not os.path.exists(path):
         return None
     m = re.search(r"hbm_bytes_per_cell_update:\s*([0-9.eE+-]+)\s*\(launch shape (\d+),(\d+),(\d+)\)",
@@ -100,12 +107,12 @@ def calibration_copy(nbytes):
     log("calibration copy: %d bytes device-to-device" % nbytes)
 
 
-def run_batch(S, pbs, prims, steps, launch, K, W, progress=0):
+def run_batch(S, pbs, prims, steps, launch, K, W, progress=0, trial=2048):
     b = S.Batch(pbs, prims, steps)
     if launch:
         b.set_launch(*launch)
     else:
-        b.autotune(2048)
+        b.autotune(trial)
     if W > 0:
         b.advance(W)
         b.reset()
@@ -137,9 +144,10 @@ def updates_done(pbs, steps, K):
     return sum(p.nx * p.ny * min(K, s) for p, s in zip(pbs, steps))
 
 
-def cpu_baseline(S, pbs, prims, steps):
-    """Oracle REF32 (the reference's arithmetic and pass structure), serial, on every
-    256th tube of the batch for up to 2000 steps each."""
+def cpu_baseline(S, pbs, prims, steps, nthreads=1):
+    """Oracle REF32 (the reference's arithmetic and pass structure) on every 256th tube of
+    the batch for up to 2000 steps each: serially (the base-c loop), or with one OpenMP
+    worksharing loop per pass on nthreads threads (the base-omp pattern)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     idx = list(range(0, len(pbs), 256))
@@ -148,15 +156,26 @@ def cpu_baseline(S, pbs, prims, steps):
         pb = pbs[k]
         n = min(steps[k], 2000)
         op = oracle.problem(1, 2, pb.nx)
-        _, _, el = oracle.run(op, oracle.REF32, prims[k], n, nthreads=1, timed=True)
+        _, _, el = oracle.run(op, oracle.REF32, prims[k], n, nthreads=nthreads, timed=True)
         total += pb.nx * n
         secs += el
-    return {"value": total / secs, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+    how = "1 thread" if nthreads == 1 else "%d OpenMP threads, one worksharing loop per pass" % nthreads
+    return {"value": total / secs, "unit": "cell-updates/s", "cores": nthreads, "kind": "port",
             "sample": "%d tubes (every 256th of the batch, N=%d..%d), min(steps,2000) steps each, "
                       "%.3g cell-updates in %.1f s; oracle REF32 = reference float arithmetic and "
-                      "unfused pass structure, gcc -O3, 1 thread; host %s" % (
+                      "unfused pass structure, gcc -O3, %s; host %s" % (
                           len(idx), min(pbs[k].nx for k in idx), max(pbs[k].nx for k in idx), total, secs,
-                          cpu_model())}
+                          how, cpu_model())}
+
+
+def omp_threads():
+    """Threads for the OpenMP baseline: the CPUs this process may use, at most 16 (a
+    one-GPU share of the host)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
 
 
 def extra_single(S):
@@ -309,6 +328,92 @@ def tube_errors(S, b, pbs, judge):
     }
 
 
+DOMAIN_SEED = 4242
+NDOMAINS = 64
+DOMAIN_STEPS = 1000
+
+
+def cpu_baseline_2d(pbs, prims, nthreads=1, steps=40):
+    """Oracle REF32 first-order 2D (the reference's arithmetic and pass structure) on every
+    8th domain of the batch for a few steps each."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    idx = list(range(0, len(pbs), 8))
+    total, secs = 0, 0.0
+    for k in idx:
+        pb = pbs[k]
+        op = oracle.problem(2, 1, pb.nx, pb.ny, lx=pb.lx, ly=pb.ly)
+        _, _, el = oracle.run(op, oracle.REF32, prims[k], steps, nthreads=nthreads, timed=True)
+        total += pb.nx * pb.ny * steps
+        secs += el
+    how = "1 thread" if nthreads == 1 else "%d OpenMP threads, one worksharing loop per pass" % nthreads
+    return {"value": total / secs, "unit": "cell-updates/s", "cores": nthreads, "kind": "port",
+            "sample": "%d domains (every 8th of the batch), %d steps each, %.3g cell-updates in %.1f s; oracle "
+                      "REF32 = reference float arithmetic and unfused pass structure, gcc -O3, %s; host %s" % (
+                          len(idx), steps, total, secs, how, cpu_model())}
+
+
+def main_domains(S, args):
+    """64 uneven first-order 2D domains (side lengths log-uniform in [256, 2048], 1-4
+    circles each), 1000 steps, FP32-compensated, one launch per k steps over the
+    (domain, tile) work list; the FP64 batch of the same domains judges accuracy."""
+    form = S.PREC_NAMES[args.form]
+    pbs, prims = S.domain_batch(DOMAIN_SEED, NDOMAINS, precision=form)
+    steps = [DOMAIN_STEPS] * NDOMAINS
+    K = args.steps if args.steps > 0 else DOMAIN_STEPS
+    launch = tuple(int(x) for x in args.launch.split(",")) if args.launch else None
+    b, wall, avg_ms, launches, ev_ms = run_batch(S, pbs, prims, steps, launch, K, args.warmup, args.progress, 256)
+    upd = updates_done(pbs, steps, K)
+    lau = list(b.get_launch())
+    B = bytes_per_update(2, form)
+    achieved = B * upd / (ev_ms * 1e-3) / 1e9 if ev_ms > 0 else 0.0
+    log("timed %d steps: %.3e cell-updates in %.3f s (events %.1f ms, %d launches)" % (K, upd, wall, ev_ms, launches))
+    acc = None
+    if not args.no_accuracy:
+        ref = S.Batch([S.problem(2, 1, p.nx, p.ny, precision=S.F64, lx=p.lx, ly=p.ly) for p in pbs], prims,
+                      [K] * NDOMAINS)
+        t0 = time.perf_counter()
+        ref.run()
+        dt64 = time.perf_counter() - t0
+        errs = [b.compare(k, ref)["l1_rho"] / (pbs[k].lx * pbs[k].ly) for k in range(NDOMAINS)]
+        acc = {"judge": "FP64 batch of the same domains (its launch shape %s, %.3e cell-updates/s)" % (
+                   list(ref.get_launch()), upd / dt64),
+               "l1_rho_per_unit_area_max": max(errs), "l1_rho_per_unit_area_median": sorted(errs)[NDOMAINS // 2],
+               "nonfinite_domains": int((b.check_state() > 0).sum() + (ref.check_state() > 0).sum())}
+        ref.close()
+    b.close()
+    tpu = traffic_per_update(lau, args.form, "k_step2d_tb_%s_domains64_hbm_traffic.md" % args.form)
+    tr = tpu * upd / launches if tpu is not None and launches > 0 else None
+    names = {S.C32: "exact TwoSum face update", S.C32K: "Kahan-compensated update"}
+    result = {
+        "metric": METRIC, "value": upd / wall, "unit": "cell-updates/s", "n_gpus": 1, "steps": K,
+        "warmup": args.warmup, "ms_per_step": wall * 1e3 / K, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "batched 2D first-order SHLL domains, FP32 fluxes + (hi,lo) FP32 state, " + names[form],
+                   "domains": NDOMAINS, "seed": DOMAIN_SEED, "side_range": [256, 2048],
+                   "cells": int(sum(p.nx * p.ny for p in pbs)), "cell_updates_timed": int(upd),
+                   "launches": launches, "parallelism": "1 GPU",
+                   "launch_shape": {"steps_per_launch": lau[1], "waves_per_block": lau[2], "graph_steps": lau[3]}},
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": achieved / PEAK_HBM_GBS, "traffic": tr,
+            "kernel": "shll::k_step2d_tb<%d, %d, %d> (FP32-compensated, first order, %d steps per launch)" % (
+                form, lau[1], lau[2], lau[1]) if lau[1] > 1 else "shll::k_step2d<%d, 1, %d, 0>" % (form, lau[2]),
+            "algorithmic_bytes_per_cell_update": B, "avg_launch_ms": avg_ms,
+            "note": "achieved = 64 B x cell-updates / HIP-event time of the launches (k steps per launch read "
+                    "and write the state once). traffic = calibrated rocprofv3 FETCH_SIZE+WRITE_SIZE per "
+                    "cell-update (profiles/k_step2d_tb_%s_domains64_hbm_traffic.md) x cell-updates per launch"
+                    % args.form},
+        "accuracy": acc,
+    }
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline_2d(pbs, prims)
+        result["cpu_baseline_omp"] = cpu_baseline_2d(pbs, prims, nthreads=omp_threads())
+    else:
+        result["cpu_baseline"] = None
+    print(json.dumps(result))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -325,6 +430,9 @@ def main():
     ap.add_argument("--autotune", action="store_true", help="re-run the autotuner instead of its recorded choices")
     ap.add_argument("--form", default="c32k", choices=["c32k", "c32"],
                     help="compensated update of the timed batch: Kahan (c32k) or exact TwoSum (c32)")
+    ap.add_argument("--workload", default="tubes", choices=["tubes", "domains"],
+                    help="tubes: the 4096-tube second-order batch (headline); domains: 64 uneven 2D "
+                         "first-order domains, 1000 steps")
     args = ap.parse_args()
     if not args.autotune:
         os.environ.setdefault("SHLL_TUNE_CACHE", TUNE_CACHE)
@@ -335,6 +443,9 @@ def main():
 
     if args.calib_copy > 0:
         calibration_copy(args.calib_copy)
+    if args.workload == "domains":
+        main_domains(S, args)
+        return
     form = S.PREC_NAMES[args.form]
     other = S.C32 if form == S.C32K else S.C32K
     names = {S.C32: "FP32 fluxes + (hi,lo) FP32 state, exact TwoSum face update",
@@ -397,7 +508,7 @@ def main():
             "unit": "GB/s",
             "frac": achieved / PEAK_HBM_GBS,
             "traffic": tr,
-            "kernel": "shll::k_step1d<%d, 2, %d, %d> (FP32-compensated %s, second order, %d cells/lane, %d waves/block)" % (
+            "kernel": "shll::k_step1d<%d, 2, %d, %d, 0> (FP32-compensated %s, second order minmod, %d cells/lane, %d waves/block)" % (
                 form, lau[0], lau[2], "Kahan" if form == S.C32K else "TwoSum", lau[0], lau[2]),
             "algorithmic_bytes_per_cell_update": B,
             "avg_launch_ms": avg_ms,
@@ -410,6 +521,7 @@ def main():
     }
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(S, pbs, prims, steps)
+        result["cpu_baseline_omp"] = cpu_baseline(S, pbs, prims, steps, nthreads=omp_threads())
     else:
         result["cpu_baseline"] = None
     if not args.no_extra:

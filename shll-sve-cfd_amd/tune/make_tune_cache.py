@@ -38,6 +38,18 @@ def main():
         with S.Solver(pb, S.ic_tube(pb.nx)) as s:
             s.autotune(trial)
             print("single dim1 order%d prec%d N=%d:" % (pb.order, pb.precision, pb.nx), s.get_launch(), flush=True)
+    # 2D first order: the 64-domain batch (both compensated forms) and one 4096^2 domain
+    # (every precision); a 256-step trial covers 64 launches of 4 steps
+    for prec in (S.C32K, S.C32):
+        pbs, prims = S.domain_batch(4242, 64, precision=prec)
+        b = S.Batch(pbs, prims, [1000] * 64)
+        b.autotune(256)
+        print("batch dim2 order1 prec%d:" % prec, b.get_launch(), flush=True)
+        b.close()
+    for prec in (S.F64, S.C32, S.C32K):
+        with S.Solver(S.problem(2, 1, 4096, 4096, precision=prec), S.ic_circle(4096, 4096)) as s:
+            s.autotune(256)
+            print("single dim2 order1 prec%d 4096^2:" % prec, s.get_launch(), flush=True)
     print(open(out).read())
 
 
