@@ -413,3 +413,87 @@ def test_2d_batch_steps_per_launch(orc, prec, k):
         ref = oracle_run(orc, pbs[i], prims[i], steps[i])
         np.testing.assert_array_equal(b.conserved(i), ref[1], err_msg="domain %d" % i)
     b.close()
+
+
+# ---------------------------------------------------------------------------
+# edge cases: smallest grids, sizes around segment and strip edges, empty and
+# one-instance batches, large grids
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("prec", [S.F64, S.REF32, S.C32, S.C32K])
+def test_1d_smallest_and_segment_edge_sizes(orc, prec):
+    """2..5 cells (both walls in one lane) and sizes around 64*C-cell segment edges, with
+    launch shapes of 1, 2 and 8 cells per lane."""
+    banner("1d sizes", prec)
+    for order, bc in ((1, 0), (2, 1), (2, 0)):
+        for n in (2, 3, 4, 5, 63, 64, 65, 66, 127, 130, 447, 449, 513):
+            pb = S.problem(1, order, n, precision=prec, bc=bc)
+            ic = icgen.random_1d(n, 100 + n, block=3)
+            ora = oracle_run(orc, pb, ic, 21)
+            for launch in ((1, 1, 1, 0), (2, 8, 4, 0), (8, 16, 1, 32)):
+                assert_same(pb, gpu_run(pb, ic, 21, launch), ora, "n=%d o%d bc%d %s" % (n, order, bc, launch))
+
+
+@pytest.mark.parametrize("prec", [S.F64, S.REF32, S.C32, S.C32K])
+def test_2d_smallest_grids(orc, prec):
+    """Grids of 2 rows or columns and strips narrower than a wavefront, square cells (the
+    reference's DT follows DX only), first order at 1 and 4 steps per launch, second order."""
+    banner("2d sizes", prec)
+    for nx, ny in ((2, 2), (2, 64), (64, 2), (3, 70), (65, 3), (5, 130), (130, 5)):
+        h = 1.0 / max(nx, ny)
+        ic = icgen.random_2d(nx, ny, 7, block=9)
+        for order, ks in ((1, (1, 4)), (2, (1,))):
+            for bc in (0, 1):
+                pb = S.problem(2, order, nx, ny, precision=prec, bc=bc, lx=nx * h, ly=ny * h)
+                ora = oracle_run(orc, pb, ic, 9)
+                for k in ks:
+                    assert_same(pb, gpu_run(pb, ic, 9, (1, k, 4, 0)), ora,
+                                "%dx%d o%d bc%d k%d" % (nx, ny, order, bc, k))
+
+
+@pytest.mark.parametrize("dim", [1, 2])
+def test_batch_with_empty_and_tiny_instances(orc, dim):
+    """A batch holding an instance with no steps (left at its initial state), the smallest
+    instances and a single-launch one; and a batch of one instance."""
+    banner("batch edges", dim)
+    if dim == 1:
+        sizes = [2, 3, 700, 64, 5000]
+        pbs = [S.problem(1, 2, n, precision=S.C32K) for n in sizes]
+        ics = [icgen.random_1d(n, 200 + i, block=2) for i, n in enumerate(sizes)]
+        launch = (2, 8, 4, 16)
+    else:
+        sizes = [(2, 2), (3, 70), (90, 40), (64, 2), (150, 150)]
+        pbs = []
+        for nx, ny in sizes:
+            h = 1.0 / max(nx, ny)
+            pbs.append(S.problem(2, 1, nx, ny, precision=S.C32K, lx=nx * h, ly=ny * h))
+        ics = [icgen.random_2d(nx, ny, 210 + i, block=5) for i, (nx, ny) in enumerate(sizes)]
+        launch = (1, 4, 4, 8)
+    steps = [0, 17, 40, 1, 33]
+    b = S.Batch(pbs, ics, steps)
+    b.set_launch(*launch)
+    b.run()
+    for i in range(len(pbs)):
+        ref = oracle_run(orc, pbs[i], ics[i], steps[i])
+        np.testing.assert_array_equal(b.conserved(i), ref[1], err_msg="instance %d" % i)
+        np.testing.assert_array_equal(b.primitives(i), ref[0], err_msg="instance %d" % i)
+    b.close()
+    b = S.Batch(pbs[2:3], ics[2:3], [23])
+    b.run()
+    ref = oracle_run(orc, pbs[2], ics[2], 23)
+    np.testing.assert_array_equal(b.conserved(0), ref[1])
+    b.close()
+
+
+def test_large_grids(orc):
+    """A 4M-cell tube and an 8M-cell 2D grid, a few steps, against the oracle (bitwise)."""
+    n = 1 << 22
+    banner("large 1d", n)
+    pb = S.problem(1, 2, n, precision=S.C32K)
+    ic = icgen.random_1d(n, 301, block=4096)
+    assert_same(pb, gpu_run(pb, ic, 5, (8, 16, 1, 0)), oracle_run(orc, pb, ic, 5), "1d 4M")
+    nx, ny = 4096, 2048
+    ic = icgen.random_2d(nx, ny, 302, block=256)
+    for prec, k in ((S.F64, 2), (S.C32K, 4)):
+        banner("large 2d", prec, k)
+        pb = S.problem(2, 1, nx, ny, precision=prec, lx=1.0, ly=0.5)
+        assert_same(pb, gpu_run(pb, ic, 6, (1, k, 4, 0)), oracle_run(orc, pb, ic, 6), "2d 8M k%d" % k)
