@@ -137,6 +137,13 @@ shll_status shll_write_results(shll_ctx *ctx, const char *path);
  * layout with seven columns (x, rho, u, T, u0, u1, u2).  Float values, "%e" tab-separated. */
 shll_status shll_write_results_full(shll_ctx *ctx, const char *path);
 
+/* Stage-level evaluation for checks against the reference's stage functions: from the
+ * context's current state, the sound speed a of Compute_P_from_U and the split fluxes of
+ * Compute_F_from_P (base_shll.c:130-177, base_shll_2d.c:239-319), computed exactly as the
+ * step kernels compute them (compensated forms: from the hi word), as doubles.  a: nx*ny;
+ * fp, fm (x) and hp, hm (y, 2D only): nvar*nx*ny each, SoA.  Any output may be NULL. */
+shll_status shll_stage_fluxes(shll_ctx *ctx, double *a, double *fp, double *fm, double *hp, double *hm);
+
 /* FP64 sums of the conserved variables times the cell volume (mass, momentum(s),
  * energy); out has nvar entries.  No reference counterpart (diagnostic). */
 shll_status shll_conserved_sums(shll_ctx *ctx, double *out);

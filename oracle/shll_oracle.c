@@ -106,6 +106,7 @@ typedef struct {
     float R, gR, rcv, lamx, lamy, hlamx, hlamy, theta;
     double cv64, gR64;
     float *hi[4], *lo[4], *fp[4], *fm[4], *hp[4], *hm[4];
+    float *a;  /* sound speed of the hi state (stage output) */
     void *block;
 } c32grid;
 
@@ -149,7 +150,7 @@ static int c32_alloc(c32grid *g, const orc_problem *pb)
     g->hlamy = 0.5f * g->lamy;
     g->cv64 = k.cv;
     g->gR64 = k.gR;
-    g->block = malloc(n * sizeof(float) * (size_t)(g->nv * (pb->dim == 1 ? 4 : 6)) + 64);
+    g->block = malloc(n * sizeof(float) * (size_t)(g->nv * (pb->dim == 1 ? 4 : 6) + 1) + 64);
     if (!g->block) return -1;
     base = (float *)g->block;
     for (v = 0; v < g->nv; v++) {
@@ -159,6 +160,7 @@ static int c32_alloc(c32grid *g, const orc_problem *pb)
         g->fm[v] = base; base += n;
         if (pb->dim == 2) { g->hp[v] = base; base += n; g->hm[v] = base; base += n; }
     }
+    g->a = base;
     return 0;
 }
 
@@ -240,6 +242,7 @@ static void c32_cell_flux(c32grid *g, int c)
         float a = sqrtf(g->gR * T);
         float z1, z2, z3;
         float P = (r * g->R) * T;
+        g->a[c] = a;
         float f[3], U[3];
         c32_z(vx / a, a, &z1, &z2, &z3);
         f[0] = m; f[1] = fmaf(m, vx, P); f[2] = vx * (E + P);
@@ -253,6 +256,7 @@ static void c32_cell_flux(c32grid *g, int c)
         float a = sqrtf(g->gR * T);
         float P = (r * g->R) * T;
         float U[4], f[4], z1, z2, z3;
+        g->a[c] = a;
         U[0] = r; U[1] = mx; U[2] = my; U[3] = E;
         c32_z(vx / a, a, &z1, &z2, &z3);
         f[0] = mx; f[1] = fmaf(mx, vx, P); f[2] = mx * vy; f[3] = vx * (E + P);
@@ -480,8 +484,9 @@ int orc_stage(const orc_problem *pb, int mode, const double *prim_in, double *co
                 if (hp && g.dim == 2) hp[o] = g.hp[v][c];
                 if (hm && g.dim == 2) hm[o] = g.hm[v][c];
             }
+        if (a)
+            for (c = 0; c < g.n; c++) a[c] = g.a[c];
         if (prim) c32_output(&g, prim, NULL);
-        (void)a;
         free(g.block);
     } else {
         return -1;

@@ -96,7 +96,7 @@ int orc_run_c32_raw(const orc_problem *pb, int mode, const double *prim_in, int6
 /* Stage evaluation on whole arrays: the reference's Compute_U_from_P, Compute_P_from_U and
  * Compute_F_from_P applied once to prim_in (base_shll.c:74-177).  Any output may be NULL.
  *   cons : nvar planes (C32: hi+lo),  prim : nvar planes after the P<-U pass,
- *   a    : sound speed (one plane; not produced in C32),
+ *   a    : sound speed (one plane; C32: of the hi state, in float),
  *   fp/fm: x split fluxes (nvar planes), hp/hm: y split fluxes (2D only). */
 int orc_stage(const orc_problem *pb, int mode, const double *prim_in, double *cons, double *prim,
               double *a, double *fp, double *fm, double *hp, double *hm);

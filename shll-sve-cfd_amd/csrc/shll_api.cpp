@@ -27,6 +27,8 @@ hipError_t launch_step2d(int prec, int order, int lim, int k, int wpb, int n_ite
 bool step2d_supported(int order, int k);
 hipError_t launch_init(int prec, int dim, int nblocks, int n_inst, hipStream_t st, const Inst* insts,
                        const double* prim, void* o0, void* o1, int64_t plane);
+hipError_t launch_stage(int prec, int dim, int nblocks, hipStream_t st, const Inst* inst, const void* s0,
+                        int64_t plane, double* out);
 hipError_t launch_output(int prec, int dim, int nblocks, int n_inst, hipStream_t st, const Inst* insts,
                          const void* s0, const void* s1, int64_t plane, double* prim, double* cons);
 hipError_t launch_sums(int prec, int dim, int nblocks, int n_inst, hipStream_t st, const Inst* insts,
@@ -935,6 +937,30 @@ extern "C" shll_status shll_check_state(shll_ctx* ctx, int64_t* bad_cells) {
     const int64_t bad = (int64_t)s[4];
     if (bad_cells) *bad_cells = bad;
     if (bad > 0) return fail(SHLL_ENONFINITE, "%lld cells with non-finite or non-positive density/energy", (long long)bad);
+    return SHLL_OK;
+}
+
+extern "C" shll_status shll_stage_fluxes(shll_ctx* ctx, double* a, double* fp, double* fm, double* hp, double* hm) {
+    if (!ctx) return fail(SHLL_EINVAL, "ctx is NULL");
+    Engine* E = ctx->E;
+    if (E->dim == 1 && (hp || hm)) return fail(SHLL_EINVAL, "y fluxes requested for a 1D problem");
+    HIPCHECK(hipSetDevice(E->device));
+    const int64_t nc = E->inst_cells[0], P = E->plane;
+    double* d = nullptr;
+    HIPCHECK(hipMalloc(&d, (size_t)(1 + 4 * E->nv) * P * sizeof(double)));
+    hipError_t e = launch_stage(E->prec, E->dim, elem_blocks(nc), E->stream, E->d_inst, E->buf[E->cur][0], P, d);
+    double* outs[5] = {a, fp, fm, hp, hm};
+    for (int k = 0; k < 5 && e == hipSuccess; ++k) {
+        if (!outs[k]) continue;
+        const int nv = k == 0 ? 1 : E->nv;
+        const double* src = k == 0 ? d : d + P + (size_t)(k - 1) * E->nv * P;
+        for (int v = 0; v < nv && e == hipSuccess; ++v)
+            e = hipMemcpyAsync(outs[k] + (size_t)v * nc, src + (size_t)v * P, nc * sizeof(double),
+                               hipMemcpyDeviceToHost, E->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(E->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(SHLL_EDEVICE, "stage evaluation: %s", hipGetErrorString(e));
     return SHLL_OK;
 }
 

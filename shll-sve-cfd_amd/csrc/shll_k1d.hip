@@ -14,14 +14,9 @@
 // References: base_shll.c:87-177 (first order, reflective), 2nd_order_base_shll.c
 // :203-459 (second order, outflow; its x sweep on the 3-variable system is the 1D
 // second-order scheme), base-omp/2nd_order_base_shll.c:317-325 (the MC limiter).
-#include "shll_device.h"
+#include "shll_flux.h"
 
 namespace shll {
-
-template <int PREC> struct Real { using T = double; };
-template <> struct Real<PREC_REF32> { using T = float; };
-template <> struct Real<PREC_C32> { using T = float; };
-template <> struct Real<PREC_C32K> { using T = float; };
 
 // Wall modes of a wavefront's segment (a wave-uniform choice, see k_step1d):
 //   WM_NONE   the segment is at least 3 cells from both ends: no boundary rule applies
@@ -98,8 +93,6 @@ template <int PREC, int ORDER, int C, int WM, int LIM>
 __device__ __forceinline__ void step_ref(typename Real<PREC>::T (&u)[3][C], const Inst& in, int gl) {
     using RT = typename Real<PREC>::T;
     using A = RefArith<RT>;
-    const RT cv = PREC == PREC_F64 ? (RT)in.k.d_cv : (RT)in.k.f_cv;
-    const RT gR = PREC == PREC_F64 ? (RT)in.k.d_gR : (RT)in.k.f_gR;
     const RT lam = PREC == PREC_F64 ? (RT)in.k.d_lamx : (RT)in.k.f_lamx;
     const RT th = PREC == PREC_F64 ? (RT)in.k.d_theta : (RT)in.k.f_theta;
     const int n = in.nx;
@@ -108,14 +101,13 @@ __device__ __forceinline__ void step_ref(typename Real<PREC>::T (&u)[3][C], cons
     RT fp[3][C], fm[3][C];
 #pragma unroll
     for (int m = 0; m < C; ++m) {
-        RT vx, T, a, z1, z2, z3;
-        A::prim1(u[0][m], u[1][m], u[2][m], in.ke, cv, gR, vx, T, a);
-        A::zinv(div_rn(vx, a), a, z1, z2, z3);
-        RT P = u[0][m] * T;  // (rho*R)*T with R = 1 (base_shll.c:17, :141)
-        RT f0 = u[1][m], f1 = u[1][m] * vx + P, f2 = vx * (u[2][m] + P);
-        split<RT>(f0, u[0][m], z1, z2, z3, fp[0][m], fm[0][m]);
-        split<RT>(f1, u[1][m], z1, z2, z3, fp[1][m], fm[1][m]);
-        split<RT>(f2, u[2][m], z1, z2, z3, fp[2][m], fm[2][m]);
+        RT p[3], q[3], a;
+        flux1<PREC, RT>(u[0][m], u[1][m], u[2][m], in, p, q, a);
+#pragma unroll
+        for (int v = 0; v < 3; ++v) {
+            fp[v][m] = p[v];
+            fm[v][m] = q[v];
+        }
     }
 #pragma unroll
     for (int v = 0; v < 3; ++v) {
@@ -149,20 +141,19 @@ __device__ __forceinline__ void step_ref(typename Real<PREC>::T (&u)[3][C], cons
 template <int ORDER, int C, int WM, bool KAHAN, int LIM>
 __device__ __forceinline__ void step_c32(float (&hi)[3][C], float (&lo)[3][C], const Inst& in, int gl) {
     using A = C32Arith;
-    const float rcv = in.k.c_rcv, gR = in.k.c_gR, lam = in.k.c_lamx, hlam = 0.5f * lam, th = in.k.c_theta;
+    const float lam = in.k.c_lamx, hlam = 0.5f * lam, th = in.k.c_theta;
     const int n = in.nx;
     if constexpr (WM != WM_NONE) ghost_cells<C>(hi, gl, n, in.bc == BC_REFLECT);
     float fp[3][C], fm[3][C];
 #pragma unroll
     for (int m = 0; m < C; ++m) {
-        float vx, T, a, z1, z2, z3;
-        A::prim1(hi[0][m], hi[1][m], hi[2][m], rcv, gR, vx, T, a);
-        A::zinv(div_fast(vx, a), a, z1, z2, z3);
-        float P = hi[0][m] * T;  // R = 1
-        float f0 = hi[1][m], f1 = __builtin_fmaf(hi[1][m], vx, P), f2 = vx * (hi[2][m] + P);
-        split_c32(f0, hi[0][m], z1, z2, z3, fp[0][m], fm[0][m]);
-        split_c32(f1, hi[1][m], z1, z2, z3, fp[1][m], fm[1][m]);
-        split_c32(f2, hi[2][m], z1, z2, z3, fp[2][m], fm[2][m]);
+        float p[3], q[3], a;
+        flux1<PREC_C32, float>(hi[0][m], hi[1][m], hi[2][m], in, p, q, a);
+#pragma unroll
+        for (int v = 0; v < 3; ++v) {
+            fp[v][m] = p[v];
+            fm[v][m] = q[v];
+        }
     }
 #pragma unroll
     for (int v = 0; v < 3; ++v) {

@@ -14,7 +14,7 @@
 // 2nd_order_base_shll.c:436-458).  Walls: reflective with per-component signs
 // (base_shll_2d.c:150-216) or outflow (2nd_order_base_shll.c:214-323).  Second-order
 // slopes: minmod or the MC limiter of base-omp/2nd_order_base_shll.c:317-325,385-392.
-#include "shll_device.h"
+#include "shll_flux.h"
 
 #ifndef PF_DEPTH
 #define PF_DEPTH 2
@@ -22,59 +22,13 @@
 
 namespace shll {
 
-template <int PREC> struct Real2 { using T = double; };
-template <> struct Real2<PREC_REF32> { using T = float; };
-template <> struct Real2<PREC_C32> { using T = float; };
-template <> struct Real2<PREC_C32K> { using T = float; };
-
-// x and y split fluxes of one cell.  U: conserved (F64/REF32: the state; C32: hi).
-template <int PREC, class RT>
-__device__ __forceinline__ void fluxes2(const RT (&U)[4], const Inst& in, RT (&xp)[4], RT (&xm)[4], RT (&yp)[4],
-                                        RT (&ym)[4]) {
-    RT vx, vy, T, a, z1, z2, z3;
-    if constexpr (PREC == PREC_C32 || PREC == PREC_C32K) {
-        C32Arith::prim2(U[0], U[1], U[2], U[3], in.k.c_rcv, in.k.c_gR, vx, vy, T, a);
-    } else {
-        const RT cv = PREC == PREC_F64 ? (RT)in.k.d_cv : (RT)in.k.f_cv;
-        const RT gR = PREC == PREC_F64 ? (RT)in.k.d_gR : (RT)in.k.f_gR;
-        RefArith<RT>::prim2(U[0], U[1], U[2], U[3], cv, gR, vx, vy, T, a);
-    }
-    const RT P = U[0] * T;  // (rho*R)*T with R = 1 (base_shll_2d.c:24, :246)
-    if constexpr (PREC == PREC_C32 || PREC == PREC_C32K) {
-        // the compensated scheme's float form: fused multiply-adds (split_c32)
-        C32Arith::zinv(div_fast(vx, a), a, z1, z2, z3);
-        split_c32(U[1], U[0], z1, z2, z3, xp[0], xm[0]);
-        split_c32(__builtin_fmaf(U[1], vx, P), U[1], z1, z2, z3, xp[1], xm[1]);
-        split_c32(U[1] * vy, U[2], z1, z2, z3, xp[2], xm[2]);
-        split_c32(vx * (U[3] + P), U[3], z1, z2, z3, xp[3], xm[3]);
-        C32Arith::zinv(div_fast(vy, a), a, z1, z2, z3);
-        split_c32(U[2], U[0], z1, z2, z3, yp[0], ym[0]);
-        split_c32(U[2] * vx, U[1], z1, z2, z3, yp[1], ym[1]);
-        split_c32(__builtin_fmaf(U[2], vy, P), U[2], z1, z2, z3, yp[2], ym[2]);
-        split_c32(vy * (U[3] + P), U[3], z1, z2, z3, yp[3], ym[3]);
-    } else {
-        // x direction (base_shll_2d.c:249-272)
-        RefArith<RT>::zinv(div_rn(vx, a), a, z1, z2, z3);
-        split<RT>(U[1], U[0], z1, z2, z3, xp[0], xm[0]);
-        split<RT>(U[1] * vx + P, U[1], z1, z2, z3, xp[1], xm[1]);
-        split<RT>(U[1] * vy, U[2], z1, z2, z3, xp[2], xm[2]);
-        split<RT>(vx * (U[3] + P), U[3], z1, z2, z3, xp[3], xm[3]);
-        // y direction (base_shll_2d.c:275-298)
-        RefArith<RT>::zinv(div_rn(vy, a), a, z1, z2, z3);
-        split<RT>(U[2], U[0], z1, z2, z3, yp[0], ym[0]);
-        split<RT>(U[2] * vx, U[1], z1, z2, z3, yp[1], ym[1]);
-        split<RT>(U[2] * vy + P, U[2], z1, z2, z3, yp[2], ym[2]);
-        split<RT>(vy * (U[3] + P), U[3], z1, z2, z3, yp[3], ym[3]);
-    }
-}
-
 // reflective wall signs: x walls (-,+,-,-), y walls (-,-,+,-)
 __device__ __forceinline__ int sgn_x(int v) { return v == 1 ? 1 : -1; }
 __device__ __forceinline__ int sgn_y(int v) { return v == 2 ? 1 : -1; }
 
 template <int PREC, int ORDER, int WPB, int LIM>
 __global__ __launch_bounds__(64 * WPB) void k_step2d(StepArgs a) {
-    using RT = typename Real2<PREC>::T;
+    using RT = typename Real<PREC>::T;
     using A = typename std::conditional<PREC == PREC_C32 || PREC == PREC_C32K, C32Arith, RefArith<RT>>::type;
     constexpr int NW = 2 * ORDER + 1;  // rows of x fluxes kept: c-ORDER .. c+ORDER
     constexpr int NS = ORDER + 1;      // rows of state / y fluxes kept: c .. c+ORDER
@@ -274,7 +228,7 @@ __global__ __launch_bounds__(64 * WPB) void k_step2d(StepArgs a) {
 // ---------------------------------------------------------------------------
 template <int PREC>
 struct Held2 {
-    using RT = typename Real2<PREC>::T;
+    using RT = typename Real<PREC>::T;
     RT u[4], l[4];    // state (C32: hi, lo)
     RT fp[4];         // x split flux F+ (the next row's lower face)
     RT d[4];          // F64/REF32: F+ - F- of the row
@@ -294,11 +248,11 @@ enum { ST_TAKE_FIRST = 0, ST_TAKE = 1, ST_DRAIN = 2 };
 // One stage, one row.  io: the incoming row (ST_TAKE_FIRST / ST_TAKE); on return the
 // outgoing row (ST_TAKE / ST_DRAIN).  act: the stage advances (else it only delays).
 template <int PREC, bool YW>
-__device__ __forceinline__ void stage2(Held2<PREC>& h, typename Real2<PREC>::T (&iou)[4],
-                                       typename Real2<PREC>::T (&iol)[4], int mode, bool act, const Inst& in,
-                                       const LaneWalls& w, typename Real2<PREC>::T lamx,
-                                       typename Real2<PREC>::T lamy) {
-    using RT = typename Real2<PREC>::T;
+__device__ __forceinline__ void stage2(Held2<PREC>& h, typename Real<PREC>::T (&iou)[4],
+                                       typename Real<PREC>::T (&iol)[4], int mode, bool act, const Inst& in,
+                                       const LaneWalls& w, typename Real<PREC>::T lamx,
+                                       typename Real<PREC>::T lamy) {
+    using RT = typename Real<PREC>::T;
     constexpr bool CMP = PREC == PREC_C32 || PREC == PREC_C32K;
     const bool refl = w.refl;
     if (!act) {
@@ -394,7 +348,7 @@ __device__ __forceinline__ void stage2(Held2<PREC>& h, typename Real2<PREC>::T (
 
 template <int PREC, int K, bool YW>
 __device__ __forceinline__ void pipeline2(const StepArgs& a, const Item2& it, const Inst& in, int lane, int ns) {
-    using RT = typename Real2<PREC>::T;
+    using RT = typename Real<PREC>::T;
     constexpr bool CMP = PREC == PREC_C32 || PREC == PREC_C32K;
     const int nx = in.nx, ny = in.ny;
     const int64_t P = a.plane;

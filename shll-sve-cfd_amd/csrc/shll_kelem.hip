@@ -5,10 +5,13 @@
 //                                          Save_Results prints, base_shll.c:161-192)
 //   k_sums    : FP64 conservation sums and a non-finite / non-positive census
 //   k_compare : FP64 L1 / Linf / max-relative differences of two primitive sets
+//   k_stage   : sound speed and split fluxes of every cell from the state, as the step
+//               kernels compute them (Compute_P_from_U + Compute_F_from_P,
+//               base_shll.c:130-177, base_shll_2d.c:239-319), for stage-level checks
 // Reductions: per-lane sums in a fixed cell order, a wave64 xor-butterfly
 // (__shfl_xor), then the waves of a block in index order; blocks write partials
 // that the host adds in index order, so results are deterministic.
-#include "shll_device.h"
+#include "shll_flux.h"
 
 namespace shll {
 
@@ -249,6 +252,69 @@ hipError_t launch_init(int prec, int dim, int nblocks, int n_inst, hipStream_t s
     if (prec == PREC_F64) return init_p<PREC_F64>(dim, g, st, insts, prim, o0, o1, plane);
     if (prec == PREC_REF32) return init_p<PREC_REF32>(dim, g, st, insts, prim, o0, o1, plane);
     return init_p<PREC_C32>(dim, g, st, insts, prim, o0, o1, plane);
+}
+
+// sound speed a and split fluxes (x: fp, fm; y: hp, hm) of every cell of one instance;
+// out holds 1 + 4*nvar planes of `plane` doubles: a, fp, fm, hp, hm
+template <int PREC, int DIM>
+__global__ __launch_bounds__(256) void k_stage(const Inst* insts, const void* s0, int64_t plane,
+                                               double* __restrict__ out) {
+    using RT = typename Real<PREC>::T;
+    constexpr int NV = DIM + 2;
+    const Inst in = insts[blockIdx.y];
+    const int64_t ncell = (int64_t)in.nx * in.ny;
+    const RT* u = (const RT*)s0;  // the state; the hi word in the compensated forms
+    double* a = out;
+    double* fp = out + plane;
+    double* fm = fp + NV * plane;
+    double* hp = fm + NV * plane;
+    double* hm = hp + NV * plane;
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < ncell;
+         c += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t o = in.off + c;
+        RT aa;
+        if constexpr (DIM == 1) {
+            RT p[3], q[3];
+            flux1<PREC, RT>(u[o], u[plane + o], u[2 * plane + o], in, p, q, aa);
+#pragma unroll
+            for (int v = 0; v < 3; ++v) {
+                fp[v * plane + o] = p[v];
+                fm[v * plane + o] = q[v];
+            }
+        } else {
+            const RT U[4] = {u[o], u[plane + o], u[2 * plane + o], u[3 * plane + o]};
+            RT xp[4], xm[4], yp[4], ym[4];
+            fluxes2<PREC, RT>(U, in, xp, xm, yp, ym, aa);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                fp[v * plane + o] = xp[v];
+                fm[v * plane + o] = xm[v];
+                hp[v * plane + o] = yp[v];
+                hm[v * plane + o] = ym[v];
+            }
+        }
+        a[o] = aa;
+    }
+}
+
+template <int PREC>
+static hipError_t stage_p(int dim, dim3 g, hipStream_t st, const Inst* insts, const void* s0, int64_t plane,
+                          double* out) {
+    if (dim == 1) hipLaunchKernelGGL((k_stage<PREC, 1>), g, dim3(256), 0, st, insts, s0, plane, out);
+    else hipLaunchKernelGGL((k_stage<PREC, 2>), g, dim3(256), 0, st, insts, s0, plane, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_stage(int prec, int dim, int nblocks, hipStream_t st, const Inst* inst, const void* s0,
+                        int64_t plane, double* out) {
+    dim3 g(nblocks, 1);
+    switch (prec) {
+    case PREC_F64: return stage_p<PREC_F64>(dim, g, st, inst, s0, plane, out);
+    case PREC_REF32: return stage_p<PREC_REF32>(dim, g, st, inst, s0, plane, out);
+    case PREC_C32: return stage_p<PREC_C32>(dim, g, st, inst, s0, plane, out);
+    case PREC_C32K: return stage_p<PREC_C32K>(dim, g, st, inst, s0, plane, out);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 template <int PREC>

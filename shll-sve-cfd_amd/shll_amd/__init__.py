@@ -67,7 +67,7 @@ class DomainSpec(ctypes.Structure):
 EXPORTS = [
     "shll_init", "shll_step", "shll_run_to_time", "shll_steps_to_time", "shll_get_primitives",
     "shll_get_conserved", "shll_state_bytes", "shll_get_state_raw", "shll_set_state_raw", "shll_write_results",
-    "shll_write_results_full",
+    "shll_write_results_full", "shll_stage_fluxes",
     "shll_conserved_sums", "shll_check_state", "shll_compare", "shll_compare_host", "shll_free",
     "shll_set_launch", "shll_get_launch", "shll_autotune", "shll_set_timing", "shll_last_timing",
     "shll_batch_init", "shll_batch_advance", "shll_batch_run", "shll_batch_get_primitives",
@@ -106,6 +106,7 @@ def lib():
         "shll_write_results": (c, [vp, ctypes.c_char_p]),
         "shll_write_results_full": (c, [vp, ctypes.c_char_p]),
         "shll_conserved_sums": (c, [vp, dp]),
+        "shll_stage_fluxes": (c, [vp, dp, dp, dp, dp, dp]),
         "shll_check_state": (c, [vp, ctypes.POINTER(i64)]),
         "shll_compare": (c, [vp, vp, ctypes.POINTER(Errors)]),
         "shll_compare_host": (c, [vp, dp, ctypes.POINTER(Errors)]),
@@ -258,6 +259,18 @@ class Solver:
         out = np.zeros(4)
         _check(lib().shll_conserved_sums(self.h, _dp(out)))
         return out[: self.nv]
+
+    def stage_fluxes(self):
+        """Sound speed and split fluxes of the current state, as the step kernels compute
+        them: {"a": (n,), "fp", "fm" (and "hp", "hm" in 2D): (nvar, n)}."""
+        n = self.pb.nx * self.pb.ny
+        out = {"a": np.zeros(n), "fp": np.zeros((self.nv, n)), "fm": np.zeros((self.nv, n))}
+        if self.pb.dim == 2:
+            out["hp"], out["hm"] = np.zeros((self.nv, n)), np.zeros((self.nv, n))
+        _check(lib().shll_stage_fluxes(self.h, _dp(out["a"]), _dp(out["fp"]), _dp(out["fm"]),
+                                       _dp(out["hp"]) if "hp" in out else None,
+                                       _dp(out["hm"]) if "hm" in out else None))
+        return out
 
     def check_state(self):
         bad = ctypes.c_int64(0)

@@ -497,3 +497,20 @@ def test_large_grids(orc):
         banner("large 2d", prec, k)
         pb = S.problem(2, 1, nx, ny, precision=prec, lx=1.0, ly=0.5)
         assert_same(pb, gpu_run(pb, ic, 6, (1, k, 4, 0)), oracle_run(orc, pb, ic, 6), "2d 8M k%d" % k)
+
+
+@pytest.mark.parametrize("prec", [S.F64, S.REF32, S.C32, S.C32K])
+def test_stage_fluxes_match_the_reference_stages(orc, prec):
+    """Stage level: the sound speed and split fluxes the step kernels compute from a state
+    equal the oracle's Compute_P_from_U + Compute_F_from_P (bitwise), 1D in both
+    kinetic-energy associations and 2D, on rough random states."""
+    banner("stage", prec)
+    for pb, ic in ((S.problem(1, 1, 777, precision=prec), icgen.random_1d(777, 401, block=1)),
+                   (S.problem(1, 2, 777, precision=prec), icgen.random_1d(777, 402, block=1)),
+                   (S.problem(2, 1, 90, 70, precision=prec), icgen.random_2d(90, 70, 403, block=1))):
+        with S.Solver(pb, ic) as s:
+            got = s.stage_fluxes()
+        want = orc.stage(opb(orc, pb), MODES[prec], ic)
+        for key in got:
+            np.testing.assert_array_equal(got[key], want[key].reshape(got[key].shape),
+                                          err_msg="%s dim%d order%d" % (key, pb.dim, pb.order))
