@@ -178,8 +178,20 @@ def omp_threads():
     return max(1, min(16, n))
 
 
-def extra_single(S):
-    """The single-instance configurations, each in full, with errors against FP64."""
+def cpu_rate(pb, prim, steps):
+    """cell-updates/s of the oracle's REF32 port (reference arithmetic and pass structure,
+    gcc -O3, one core) on one problem for a given number of steps."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    op = oracle.problem(pb.dim, pb.order, pb.nx, pb.ny, bc=pb.bc, lx=pb.lx, ly=pb.ly)
+    _, _, el = oracle.run(op, oracle.REF32, prim, steps, nthreads=1, timed=True)
+    return {"value": pb.nx * pb.ny * steps / el, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+            "sample": "%d steps of the same problem, %.1f s" % (steps, el)}
+
+
+def extra_single(S, cpu=True):
+    """The single-instance configurations, each in full, with errors against FP64 and (cpu)
+    the oracle's one-core rate on a few steps of the same problem."""
     out = {}
     # 1D first order, 1024 cells, t = 0.2 (configs[0]), FP64
     pb = S.problem(1, 1, 1024, precision=S.F64)
@@ -194,6 +206,8 @@ def extra_single(S):
         dt = time.perf_counter() - t0
         s2.close()
     out["c0_1d_o1_f64_N1024"] = {"steps": n, "seconds": dt, "cell_updates_per_s": 1024 * n / dt}
+    if cpu:
+        out["c0_1d_o1_f64_N1024"]["cpu_baseline"] = cpu_rate(pb, ic, n)
     # 1D second order, 65536 cells, t = 0.2, FP64 and both FP32-compensated forms
     n1 = 65536
     ic = S.ic_tube(n1)
@@ -215,6 +229,8 @@ def extra_single(S):
                                            "launch": list(s.get_launch())}
     for name in ("c32", "c32k"):
         out["c1_1d_o2_%s_N65536" % name]["error_vs_f64"] = sol[name].compare(sol["f64"])
+    if cpu:
+        out["c1_1d_o2_f64_N65536"]["cpu_baseline"] = cpu_rate(S.problem(1, 2, n1), ic, 400)
     for s in sol.values():
         s.close()
     # 2D first order, 4096^2, 2000 steps, FP64 and FP32-compensated
@@ -235,6 +251,8 @@ def extra_single(S):
                                            "launch": list(s.get_launch())}
     for name in ("c32", "c32k"):
         out["c2_2d_o1_%s_4096sq" % name]["error_vs_f64"] = sol[name].compare(sol["f64"])
+    if cpu:
+        out["c2_2d_o1_f64_4096sq"]["cpu_baseline"] = cpu_rate(S.problem(2, 1, n2, n2), ic, 2)
     for s in sol.values():
         s.close()
     # 64 uneven 2D domains, first order, 1000 steps, both FP32-compensated
@@ -526,7 +544,7 @@ def main():
         result["cpu_baseline"] = None
     if not args.no_extra:
         result["extra"] = {"c3_other_update": alt}
-        result["extra"].update(extra_single(S))
+        result["extra"].update(extra_single(S, cpu=not args.no_cpu))
     print(json.dumps(result))
 
 
