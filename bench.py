@@ -189,6 +189,32 @@ def cpu_rate(pb, prim, steps):
             "sample": "%d steps of the same problem, %.1f s" % (steps, el)}
 
 
+def reference_cpu():
+    """The reference solvers themselves (compiled from their own sources into oracle/_ref by
+    oracle/Makefile, when that build exists), timed on one core at their compiled-in sizes:
+    wall time of the stage loop run by oracle/ref_harness.c for a fixed number of steps."""
+    import tempfile
+    runs = [("base_shll", "base-c/base_shll.c: 1D first order, 256 cells", 256, 20000),
+            ("base_shll_2d", "base-c/base_shll_2d.c: 2D first order, 256^2", 256 * 256, 200),
+            ("2nd_order_base_shll", "base-c/2nd_order_base_shll.c: 2D second order, 256^2", 256 * 256, 100),
+            ("omp_2nd_order_base_shll", "base-omp/2nd_order_base_shll.c (MC limiter, OpenMP off): 2D second "
+             "order, 1024^2", 1024 * 1024, 10)]
+    out = {}
+    for exe, what, cells, steps in runs:
+        path = os.path.join(ROOT, "oracle", "_ref", exe + ".run")
+        if not os.path.exists(path):
+            continue
+        with tempfile.TemporaryDirectory() as td:
+            t0 = time.perf_counter()
+            subprocess.run([path, "steps", os.path.join(td, "s"), "-", "0,%d" % steps], cwd=td, check=True,
+                           capture_output=True, timeout=300)
+            dt = time.perf_counter() - t0
+        out[exe] = {"value": cells * steps / dt, "unit": "cell-updates/s", "cores": 1, "kind": "reference",
+                    "sample": "%s, %d steps in %.2f s (process start, set-up and two array dumps included)"
+                              % (what, steps, dt)}
+    return out
+
+
 def extra_single(S, cpu=True):
     """The single-instance configurations, each in full, with errors against FP64 and (cpu)
     the oracle's one-core rate on a few steps of the same problem."""
@@ -545,6 +571,8 @@ def main():
     if not args.no_extra:
         result["extra"] = {"c3_other_update": alt}
         result["extra"].update(extra_single(S, cpu=not args.no_cpu))
+        if not args.no_cpu:
+            result["extra"]["cpu_reference_solvers"] = reference_cpu()
     print(json.dumps(result))
 
 
