@@ -159,7 +159,8 @@ shll_status shll_compare_host(shll_ctx *ctx, const double *ref_prim_soa, shll_er
 void shll_free(shll_ctx *ctx);
 
 /* Launch shape of the step kernels.  cells_per_lane in {1,2,4,8} (1D only),
- * steps_per_launch >= 1 (temporal blocking; 2D: 1, 2, 3, 4 or 8 in first order, 1 in second),
+ * steps_per_launch >= 1 (temporal blocking; 2D: 1, 2, 3, 4 or 8 in first order; 1, or 2 for
+ * SHLL_C32/SHLL_C32K in second order),
  * waves_per_block in {1,4}, graph_steps: steps captured per HIP graph (0 = no graph).
  * Results do not depend on the launch shape (bitwise). */
 shll_status shll_set_launch(shll_ctx *ctx, int32_t cells_per_lane, int32_t steps_per_launch,

@@ -24,7 +24,7 @@ hipError_t launch_step1d(int prec, int order, int lim, int C, int wpb, int n_ite
 bool step1d_supported(int C, int wpb);
 hipError_t launch_step2d(int prec, int order, int lim, int k, int wpb, int n_items, hipStream_t s,
                          const StepArgs& a);
-bool step2d_supported(int order, int k);
+bool step2d_supported(int prec, int order, int k);
 hipError_t launch_init(int prec, int dim, int nblocks, int n_inst, hipStream_t st, const Inst* insts,
                        const double* prim, void* o0, void* o1, int64_t plane);
 hipError_t launch_stage(int prec, int dim, int nblocks, hipStream_t st, const Inst* inst, const void* s0,
@@ -720,11 +720,13 @@ static void default_launch(Engine* E, int64_t max_cells) {
     if (E->dim == 2) {
         // First order: several steps per launch (row-pipelined stages); measured on a
         // 4096^2 domain and a 64-domain batch, the FP32 forms peak at 4 steps per launch,
-        // FP64 (twice the registers per stage) at 2.  Second order: one step per launch
-        // in 16-row items (a 4096^2 domain in 64-row items leaves ~4 wavefronts per SIMD,
-        // too few to cover the row loads).
+        // FP64 (twice the registers per stage) at 2.  Second order: C32K 2 steps per launch
+        // (7.5e10 vs 5.9e10 cell-updates/s at 4096^2); the TwoSum form's two-row stages
+        // overflow the registers (4.4e10 vs 5.3e10), so it and FP64/REF32 take one step per
+        // launch in 16-row items (a 4096^2 domain in 64-row items leaves ~4 wavefronts per
+        // SIMD, too few to cover the row loads).
         E->L.C = 1;
-        E->L.k = E->order == 1 ? (E->prec == PREC_C32 || E->prec == PREC_C32K ? 4 : 2) : 1;
+        E->L.k = E->order == 1 ? (E->prec == PREC_C32 || E->prec == PREC_C32K ? 4 : 2) : (E->prec == PREC_C32K ? 2 : 1);
         E->L.wpb = 4;
         E->L.rows = rows2d(E->L.k);
         E->L.graph = 64;
@@ -1037,8 +1039,9 @@ static shll_status set_launch(Engine* E, int32_t C, int32_t k, int32_t wpb, int3
         return fail(SHLL_EINVAL, "unsupported launch shape C=%d waves/block=%d", C, wpb);
     if (E->dim == 2 && wpb != 1 && wpb != 4) return fail(SHLL_EINVAL, "waves/block must be 1 or 4");
     if (k < 1 || graph < 0) return fail(SHLL_EINVAL, "steps per launch must be >= 1, graph steps >= 0");
-    if (E->dim == 2 && !step2d_supported(E->order, k))
-        return fail(SHLL_EINVAL, "2D steps per launch must be 1 (second order) or 1, 2, 3, 4, 8 (first order); got %d", k);
+    if (E->dim == 2 && !step2d_supported(E->prec, E->order, k))
+        return fail(SHLL_EINVAL, "2D steps per launch: 1, 2, 3, 4 or 8 in first order; 1, or 2 in the "
+                                 "FP32-compensated second order (got %d)", k);
     if (E->batch && E->launches_done != 0)
         return fail(SHLL_ESTATE, "a batch's launch shape can only change before it advances (reset first)");
     Launch old = E->L;
@@ -1157,7 +1160,7 @@ static shll_status engine_autotune(Engine* E, int32_t trial_steps, int64_t shape
     } else {
         for (int k : {1, 2, 3, 4, 8})
             for (int w : {1, 4}) {
-                if (!step2d_supported(E->order, k)) continue;
+                if (!step2d_supported(E->prec, E->order, k)) continue;
                 Launch L = start;
                 L.k = k;
                 L.wpb = w;

@@ -468,6 +468,305 @@ __global__ __launch_bounds__(64 * WPB) void k_step2d_tb(StepArgs a) {
     else pipeline2<PREC, K, false>(a, it, in, lane, ns);
 }
 
+// ---------------------------------------------------------------------------
+// Second order (FP32-compensated forms), K steps per launch: the same row pipeline with a
+// delay of two rows per stage.  The face between rows c and c+1 needs the limited slopes
+// of rows c and c+1, and the slope of row c+1 needs row c+2, so a stage advances row t-2
+// when row t arrives.  It keeps two pending rows (state, x split fluxes, y terms), the F+
+// slope of the first one and that row's lower face term.  Slopes of the domain's first
+// and last rows are zero (2nd_order_base_shll.c:227-276); at a cut the first row's slope
+// and the last row's F- slope are taken as zero, which spoils two rows per stage, like
+// the radius-2 y stencil spoils two lanes at a strip edge: items carry 2K halo rows and
+// columns.  Bitwise equal to K launches of k_step2d<prec, 2, ., LIM>.
+// ---------------------------------------------------------------------------
+template <int PREC>
+struct Held2o2 {
+    using RT = typename Real<PREC>::T;
+    RT u0[4], l0[4], u1[4], l1[4];  // pending rows: the older (0) and the newer (1)
+    RT xp0[4], xm0[4], xp1[4], xm1[4];
+    RT dfp0[4];                     // limited F+ slope of row 0
+    RT tl0[4];                      // lower face term of row 0
+    RT yb0[4], yt0[4], yb1[4], yt1[4];  // y face terms (C32K: yb = tb - tt)
+};
+
+enum { S2_FIRST = 0, S2_SECOND = 1, S2_TAKE = 2, S2_DRAIN1 = 3, S2_DRAIN2 = 4 };
+
+template <int PREC, bool YW, int LIM>
+__device__ __forceinline__ void stage2o2(Held2o2<PREC>& h, float (&iou)[4], float (&iol)[4], int mode, bool act,
+                                         int ts, const Inst& in, const LaneWalls& w, bool yedge, float lamx,
+                                         float lamy, float th) {
+    using A = C32Arith;
+    const int nx = in.nx;
+    const bool refl = w.refl;
+    const float hl = 0.5f * lamx, hly = 0.5f * lamy;
+    float nu[4], nl[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        nu[v] = iou[v];
+        nl[v] = iol[v];
+    }
+    if (!act) {
+        // pass-through with the stage's two-row delay
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const float u = h.u0[v], l = h.l0[v];
+            if (mode == S2_FIRST) {
+                h.u0[v] = nu[v]; h.l0[v] = nl[v];
+            } else if (mode == S2_SECOND) {
+                h.u1[v] = nu[v]; h.l1[v] = nl[v];
+            } else if (mode == S2_TAKE) {
+                h.u0[v] = h.u1[v]; h.l0[v] = h.l1[v]; h.u1[v] = nu[v]; h.l1[v] = nl[v];
+            } else if (mode == S2_DRAIN1) {
+                h.u0[v] = h.u1[v]; h.l0[v] = h.l1[v];
+            }
+            iou[v] = u;
+            iol[v] = l;
+        }
+        return;
+    }
+    float xp[4], xm[4], yp[4], ym[4], yb[4], yt[4];
+    const bool take = mode == S2_FIRST || mode == S2_SECOND || mode == S2_TAKE;
+    if (take) {
+        fluxes2<PREC, float>(nu, in, xp, xm, yp, ym);
+        // y face terms of the incoming row: lower face tb, upper face tt (lane above)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const float sy = (float)sgn_y(v);
+            const float hpc = yp[v], hmc = ym[v];
+            const float hpl = from_left(hpc), hpr = from_right(hpc);
+            const float hml = from_left(hmc), hmr = from_right(hmc);
+            float dhp = limited_slope<A, LIM>(hpl, hpc, hpr, th);
+            float dhm = limited_slope<A, LIM>(hml, hmc, hmr, th);
+            if constexpr (YW) {
+                dhp = yedge ? 0.0f : dhp;
+                dhm = yedge ? 0.0f : dhm;
+            }
+            const float dhpl = from_left(dhp);
+            float Gb = hpl + hmc, db = dhpl - dhm;
+            if constexpr (YW) {
+                Gb = w.lo ? (w.lo_refl ? hmc + sy * hmc : hpc + hmc) : Gb;
+                db = w.lo ? 0.0f : db;
+            }
+            const float tb = __builtin_fmaf(hly, db, lamy * Gb);
+            float tt = from_right(tb);
+            if constexpr (YW)
+                tt = w.hi ? __builtin_fmaf(hly, 0.0f, lamy * (w.hi_refl ? hpc + sy * hpc : hpc + hmc)) : tt;
+            if constexpr (PREC == PREC_C32K) {
+                yb[v] = tb - tt;
+                yt[v] = 0.0f;
+            } else {
+                yb[v] = tb;
+                yt[v] = tt;
+            }
+        }
+    }
+    if (mode == S2_FIRST) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const float sx = (float)sgn_x(v);
+            h.u0[v] = nu[v]; h.l0[v] = nl[v];
+            h.xp0[v] = xp[v]; h.xm0[v] = xm[v];
+            h.yb0[v] = yb[v]; h.yt0[v] = yt[v];
+            // the wall face below the first row (made up at a cut)
+            h.tl0[v] = __builtin_fmaf(hl, 0.0f, lamx * (refl ? xm[v] + sx * xm[v] : xp[v] + xm[v]));
+        }
+        return;
+    }
+    if (mode == S2_SECOND) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            h.u1[v] = nu[v]; h.l1[v] = nl[v];
+            h.xp1[v] = xp[v]; h.xm1[v] = xm[v];
+            h.yb1[v] = yb[v]; h.yt1[v] = yt[v];
+            h.dfp0[v] = 0.0f;  // the first row: a wall row, or a cut
+        }
+        return;
+    }
+    // advance row 0 with its upper face
+    const int r = ts - 1;  // the row whose slopes complete now (TAKE)
+    const bool bnd = r <= 0 || r >= nx - 1;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const float sx = (float)sgn_x(v);
+        float face, dfp1 = 0.0f;
+        if (mode == S2_TAKE) {
+            const float dfm1 = bnd ? 0.0f : limited_slope<A, LIM>(h.xm0[v], h.xm1[v], xm[v], th);
+            dfp1 = bnd ? 0.0f : limited_slope<A, LIM>(h.xp0[v], h.xp1[v], xp[v], th);
+            face = __builtin_fmaf(hl, h.dfp0[v] - dfm1, lamx * (h.xp0[v] + h.xm1[v]));
+        } else if (mode == S2_DRAIN1) {
+            face = __builtin_fmaf(hl, h.dfp0[v] - 0.0f, lamx * (h.xp0[v] + h.xm1[v]));
+        } else {
+            const float fpc = h.xp0[v], fmc = h.xm0[v];
+            face = __builtin_fmaf(hl, 0.0f, lamx * (refl ? fpc + sx * fpc : fpc + fmc));
+        }
+        const float tl = h.tl0[v];
+        float aa = h.u0[v], bb = h.l0[v], e;
+        if constexpr (PREC == PREC_C32K) {
+            const float y = ((tl - face) + h.yb0[v]) + bb;
+            const float s2 = aa + y;
+            e = y - (s2 - aa);
+            aa = s2;
+        } else {
+            two_sum(aa, tl, aa, e);
+            bb = bb + e;
+            two_sum(aa, -face, aa, e);
+            bb = bb + e;
+            two_sum(aa, h.yb0[v], aa, e);
+            bb = bb + e;
+            two_sum(aa, -h.yt0[v], aa, e);
+            bb = bb + e;
+            fast_two_sum(aa, bb, aa, e);
+        }
+        iou[v] = aa;
+        iol[v] = e;
+        if (mode != S2_DRAIN2) {
+            // row 1 becomes row 0
+            h.u0[v] = h.u1[v]; h.l0[v] = h.l1[v];
+            h.xp0[v] = h.xp1[v]; h.xm0[v] = h.xm1[v];
+            h.yb0[v] = h.yb1[v]; h.yt0[v] = h.yt1[v];
+            h.tl0[v] = face;
+            h.dfp0[v] = dfp1;
+        }
+        if (mode == S2_TAKE) {
+            h.u1[v] = nu[v]; h.l1[v] = nl[v];
+            h.xp1[v] = xp[v]; h.xm1[v] = xm[v];
+            h.yb1[v] = yb[v]; h.yt1[v] = yt[v];
+        }
+    }
+}
+
+template <int PREC, int K, bool YW, int LIM>
+__device__ __forceinline__ void pipeline2o2(const StepArgs& a, const Item2& it, const Inst& in, int lane, int ns) {
+    const int nx = in.nx, ny = in.ny;
+    const int64_t P = a.plane;
+    const int j = it.j0 + lane;
+    const bool jok = j >= 0 && j < ny;
+    const bool jown = j >= it.jx0 && j < it.jx0 + it.jw;
+    const int jc = jok ? j : (j < 0 ? 0 : ny - 1);
+    const float* __restrict__ i0p = (const float*)a.in0 + in.off;
+    const float* __restrict__ i1p = (const float*)a.in1 + in.off;
+    float* __restrict__ o0p = (float*)a.out0 + in.off;
+    float* __restrict__ o1p = (float*)a.out1 + in.off;
+    const float lamx = in.k.c_lamx, lamy = in.k.c_lamy, th = in.k.c_theta;
+    const bool refl = in.bc == BC_REFLECT;
+    const LaneWalls w{j == 0, j == ny - 1, j == 0 && refl, j == ny - 1 && refl, refl};
+    const bool yedge = j <= 0 || j >= ny - 1;  // zero y slopes (2nd_order_base_shll.c:293-344)
+
+    constexpr int H = 2 * K;
+    const int r0 = it.ix0 - H < 0 ? 0 : it.ix0 - H;
+    const int r1 = it.ix0 + it.iw + H > nx ? nx : it.ix0 + it.iw + H;
+    Held2o2<PREC> h[K];
+#pragma unroll
+    for (int s = 0; s < K; ++s)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            h[s].u0[v] = h[s].l0[v] = h[s].u1[v] = h[s].l1[v] = 0.0f;
+            h[s].xp0[v] = h[s].xm0[v] = h[s].xp1[v] = h[s].xm1[v] = h[s].dfp0[v] = h[s].tl0[v] = 0.0f;
+            h[s].yb0[v] = h[s].yt0[v] = h[s].yb1[v] = h[s].yt1[v] = 0.0f;
+        }
+    constexpr int D = PF_DEPTH;
+    float nu[D][4], nl[D][4];  // rows t+1 .. t+D, loaded ahead
+    auto load_row = [&](int t, float (&U)[4], float (&Ul)[4]) {
+        if (t < r1) {
+            const int64_t o = (int64_t)t * ny + jc;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const float x = i0p[v * P + o], y = i1p[v * P + o];
+                U[v] = jok ? x : (v == 0 ? 1.0f : (v == 3 ? 2.5f : 0.0f));
+                Ul[v] = jok ? y : 0.0f;
+            }
+        }
+    };
+#pragma unroll
+    for (int d = 0; d < D; ++d) load_row(r0 + d, nu[d], nl[d]);
+    // Stage s is at row t - 2s in iteration t: it takes rows [r0, r1) and drains at r1 and
+    // r1 + 1, giving row t - 2s - 2 from its third row on.  Stage K-1 gives row t - 2K.
+    const int tend = r1 + 2 * K;
+    for (int t = r0; t < tend; ++t) {
+        float U[4], Ul[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            U[v] = nu[0][v];
+            Ul[v] = nl[0][v];
+        }
+#pragma unroll
+        for (int d = 0; d + 1 < D; ++d)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                nu[d][v] = nu[d + 1][v];
+                nl[d][v] = nl[d + 1][v];
+            }
+        load_row(t + D, nu[D - 1], nl[D - 1]);
+        bool live = true;  // U holds a row at the current stage's input level
+        const bool steady = ns == K && t - 2 * (K - 1) >= r0 + 2 && t < r1;
+        if (steady) {
+#pragma unroll
+            for (int s = 0; s < K; ++s)
+                stage2o2<PREC, YW, LIM>(h[s], U, Ul, S2_TAKE, true, t - 2 * s, in, w, yedge, lamx, lamy, th);
+        } else {
+#pragma unroll
+            for (int s = 0; s < K; ++s) {
+                const int ts = t - 2 * s;
+                if (ts < r0 || ts > r1 + 1) {
+                    live = false;
+                } else if (ts >= r1) {
+                    stage2o2<PREC, YW, LIM>(h[s], U, Ul, ts == r1 ? S2_DRAIN1 : S2_DRAIN2, s < ns, ts, in, w, yedge,
+                                            lamx, lamy, th);
+                    live = true;
+                } else if (live) {
+                    const int mode = ts == r0 ? S2_FIRST : (ts == r0 + 1 ? S2_SECOND : S2_TAKE);
+                    stage2o2<PREC, YW, LIM>(h[s], U, Ul, mode, s < ns, ts, in, w, yedge, lamx, lamy, th);
+                    live = mode == S2_TAKE;
+                }
+            }
+        }
+        const int c = t - 2 * K;
+        if (live && jown && c >= it.ix0 && c < it.ix0 + it.iw) {
+            const int64_t o = (int64_t)c * ny + jc;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                o0p[v * P + o] = U[v];
+                o1p[v * P + o] = Ul[v];
+            }
+        }
+    }
+}
+
+template <int PREC, int K, int WPB, int LIM>
+__global__ __launch_bounds__(64 * WPB) void k_step2d_tb2(StepArgs a) {
+    const int wave = (int)blockIdx.x * WPB + uniform((int)threadIdx.x >> 6);
+    const LaunchView lv = launch_view(a);
+    if (wave >= lv.n_items) return;
+    const int lane = (int)threadIdx.x & 63;
+    const Item2 it = ((const Item2*)a.items)[wave];
+    const Inst in = a.inst[it.inst];
+    const int64_t left = in.nsteps - lv.step_base;
+    if (left <= 0) return;
+    const int kk = a.ksteps < K ? a.ksteps : K;
+    const int ns = left < (int64_t)kk ? (int)left : kk;
+    const bool yw = it.j0 <= 0 || it.j0 + 64 >= in.ny;
+    if (yw) pipeline2o2<PREC, K, true, LIM>(a, it, in, lane, ns);
+    else pipeline2o2<PREC, K, false, LIM>(a, it, in, lane, ns);
+}
+
+template <int PREC, int WPB, int LIM>
+static hipError_t launch2_tb2_w(int k, int nblocks, hipStream_t s, const StepArgs& a) {
+    const dim3 grid(nblocks), block(64 * WPB);
+    switch (k) {
+    case 2: hipLaunchKernelGGL((k_step2d_tb2<PREC, 2, WPB, LIM>), grid, block, 0, s, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <int PREC>
+static hipError_t launch2_tb2(int k, int lim, int wpb, int nblocks, hipStream_t s, const StepArgs& a) {
+    if (lim == LIM_MC)
+        return wpb == 1 ? launch2_tb2_w<PREC, 1, LIM_MC>(k, nblocks, s, a) : launch2_tb2_w<PREC, 4, LIM_MC>(k, nblocks, s, a);
+    return wpb == 1 ? launch2_tb2_w<PREC, 1, LIM_MINMOD>(k, nblocks, s, a)
+                    : launch2_tb2_w<PREC, 4, LIM_MINMOD>(k, nblocks, s, a);
+}
+
 template <int PREC, int WPB>
 static hipError_t launch2_tb_w(int k, int nblocks, hipStream_t s, const StepArgs& a) {
     const dim3 grid(nblocks), block(64 * WPB);
@@ -503,12 +802,21 @@ static hipError_t launch2_p(int order, int lim, int wpb, int nblocks, hipStream_
     return hipErrorInvalidValue;
 }
 
-bool step2d_supported(int order, int k) { return k == 1 || (order == 1 && (k == 2 || k == 3 || k == 4 || k == 8)); }
+bool step2d_supported(int prec, int order, int k) {
+    if (k == 1) return true;
+    if (order == 1) return k == 2 || k == 3 || k == 4 || k == 8;
+    return (prec == PREC_C32 || prec == PREC_C32K) && k == 2;
+}
 
 hipError_t launch_step2d(int prec, int order, int lim, int k, int wpb, int n_items, hipStream_t s,
                          const StepArgs& a) {
     const int nblocks = (n_items + wpb - 1) / wpb;
     if (nblocks <= 0) return hipSuccess;
+    if (k != 1 && order == 2) {
+        if (prec == PREC_C32) return launch2_tb2<PREC_C32>(k, lim, wpb, nblocks, s, a);
+        if (prec == PREC_C32K) return launch2_tb2<PREC_C32K>(k, lim, wpb, nblocks, s, a);
+        return hipErrorInvalidValue;
+    }
     if (k != 1) {
         if (order != 1 || lim != LIM_MINMOD) return hipErrorInvalidValue;
         switch (prec) {

@@ -390,12 +390,22 @@ def test_2d_steps_per_launch(orc, prec, k):
 
 
 def test_2d_steps_per_launch_rejected_in_second_order():
+    """FP64 and REF32 second order run one step per launch (their stages need twice the
+    registers); the compensated forms take 2."""
     pb = S.problem(2, 2, 64, 64)
     with S.Solver(pb, S.ic_four_shocks(64, 64)) as s:
         with pytest.raises(S.ShllError) as e:
             s.set_launch(1, 4, 4, 0)
         assert e.value.code == S.EINVAL
+        with pytest.raises(S.ShllError):
+            s.set_launch(1, 2, 4, 0)
         s.step(2)
+    pb = S.problem(2, 2, 64, 64, precision=S.C32K)
+    with S.Solver(pb, S.ic_four_shocks(64, 64)) as s:
+        with pytest.raises(S.ShllError):
+            s.set_launch(1, 4, 4, 0)
+        s.set_launch(1, 2, 4, 0)
+        s.step(3)
 
 
 @pytest.mark.parametrize("k", [2, 4, 8])
@@ -514,3 +524,34 @@ def test_stage_fluxes_match_the_reference_stages(orc, prec):
         for key in got:
             np.testing.assert_array_equal(got[key], want[key].reshape(got[key].shape),
                                           err_msg="%s dim%d order%d" % (key, pb.dim, pb.order))
+
+
+@pytest.mark.parametrize("lim", [S.LIM_MINMOD, S.LIM_MC])
+@pytest.mark.parametrize("prec", [S.C32, S.C32K])
+def test_2d_second_order_two_steps_per_launch(orc, prec, lim):
+    """Second order, compensated forms, 2 steps per launch (two-row pipeline stages): the
+    oracle's bits with both walls and limiters, several items and strips, small grids and
+    a last launch of one step."""
+    banner("2d o2 k2", prec, lim)
+    for bc, (nx, ny), wpb, graph in ((1, (150, 130), 1, 0), (0, (203, 70), 4, 8), (1, (9, 7), 4, 0),
+                                     (0, (2, 2), 1, 0), (1, (3, 70), 4, 0), (0, (70, 3), 1, 0)):
+        h = 1.0 / max(nx, ny)
+        pb = S.problem(2, 2, nx, ny, precision=prec, bc=bc, lx=nx * h, ly=ny * h, limiter=lim)
+        ic = icgen.random_2d(nx, ny, 41 + bc, block=9)
+        banner("  ", bc, nx, ny)
+        assert_same(pb, gpu_run(pb, ic, 7, (1, 2, wpb, graph)), oracle_run(orc, pb, ic, 7),
+                    "2d o2 k2 %dx%d bc%d" % (nx, ny, bc))
+    pb = S.problem(2, 2, 128, 96, precision=prec, limiter=lim)
+    ic = S.ic_four_contacts(128, 96)
+    assert_same(pb, gpu_run(pb, ic, 40, (1, 2, 4, 0)), oracle_run(orc, pb, ic, 40), "2d o2 k2 contacts")
+    pbs, prims = S.domain_batch(58, 4, smin=20, smax=120, precision=prec, order=2)
+    for p in pbs:
+        p.limiter = lim
+    steps = [9, 4, 1, 6]
+    b = S.Batch(pbs, prims, steps)
+    b.set_launch(1, 2, 4, 4)
+    b.run()
+    for i in range(len(pbs)):
+        np.testing.assert_array_equal(b.conserved(i), oracle_run(orc, pbs[i], prims[i], steps[i])[1],
+                                      err_msg="batch domain %d" % i)
+    b.close()
