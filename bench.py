@@ -268,13 +268,18 @@ def extra_single(S, cpu=True):
         s.step(20)
         s.close()
         s = S.Solver(S.problem(2, 1, n2, n2, precision=prec), ic)
+        sums0 = s.conserved_sums()
         t0 = time.perf_counter()
         s.step(2000)
         dt = time.perf_counter() - t0
+        sums1 = s.conserved_sums()
         sol[name] = s
+        # reflective walls: mass and energy are conserved (the momenta change by the wall impulse)
         out["c2_2d_o1_%s_4096sq" % name] = {"steps": 2000, "seconds": dt,
                                            "cell_updates_per_s": n2 * n2 * 2000 / dt,
-                                           "launch": list(s.get_launch())}
+                                           "launch": list(s.get_launch()),
+                                           "mass_drift": abs(sums1[0] - sums0[0]) / abs(sums0[0]),
+                                           "energy_drift": abs(sums1[3] - sums0[3]) / abs(sums0[3])}
     for name in ("c32", "c32k"):
         out["c2_2d_o1_%s_4096sq" % name]["error_vs_f64"] = sol[name].compare(sol["f64"])
     if cpu:
