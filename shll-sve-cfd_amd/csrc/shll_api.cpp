@@ -22,6 +22,7 @@
 namespace shll {
 hipError_t launch_step1d(int prec, int order, int lim, int C, int wpb, int n_items, hipStream_t s, const StepArgs& a);
 bool step1d_supported(int C, int wpb);
+int step1d_items_per_block(int C, int wpb);
 hipError_t launch_step2d(int prec, int order, int lim, int k, int wpb, int n_items, hipStream_t s,
                          const StepArgs& a);
 bool step2d_supported(int prec, int order, int k);
@@ -170,13 +171,17 @@ struct Launch {
 // side, so longer items waste less; 16 rows at one step per launch
 static int rows2d(int k) { return k <= 1 ? 16 : (k == 2 ? 32 : 64); }
 
-// 1D tiling of one instance of nx cells: segments of 64*C cells, halo order*k.  The
-// segment that holds a wall also holds that wall's ghost cell (-1 or nx), so the
-// first segment starts at cell -1; the last one always reaches past cell nx.
-static bool tile1d(int32_t inst, int64_t nx, int order, int C, int k, std::vector<Item1>& out) {
-    const int64_t S = 64LL * C;
-    if (nx + 2 <= S) {
-        out.push_back(Item1{inst, -1, 0, (int32_t)nx});
+// 1D tiling of one instance of nx cells: segments of XW*64*C cells (XW wavefronts of a
+// cooperative block, else 1), halo order*k.  The segment that holds a wall also holds
+// that wall's ghost cell (-1 or nx), so the first segment starts at cell -1; the last
+// one always reaches past cell nx.  A ghost cell takes its value from its neighbour by a
+// lane shift within one wavefront, so ghost cell nx must not be a wavefront's first
+// cell: such a segment starts one cell earlier.
+static bool tile1d(int32_t inst, int64_t nx, int order, int C, int XW, int k, std::vector<Item1>& out) {
+    const int64_t S = 64LL * C * XW;
+    auto place_end = [&](int64_t g0) { return (nx - g0) % (64LL * C) == 0 ? g0 - 1 : g0; };
+    if (nx + 2 <= S) {  // a shift by one then still fits: nx + 1 is a multiple of 64*C below S
+        out.push_back(Item1{inst, (int32_t)place_end(-1), 0, (int32_t)nx});
         return true;
     }
     const int64_t H = (int64_t)order * k;
@@ -192,8 +197,11 @@ static bool tile1d(int32_t inst, int64_t nx, int order, int C, int k, std::vecto
         x0 = w;
     }
     while (x0 < nx) {
-        const int64_t g0 = x0 - H;
         const int64_t w = std::min<int64_t>(nx - x0, W);
+        // only a segment that reaches past cell nx holds its ghost; one cell more of left
+        // halo is harmless
+        const int64_t g0 = x0 - H + S - 1 >= nx ? place_end(x0 - H) : x0 - H;
+        if (g0 + S - 1 < std::min<int64_t>(x0 + w - 1 + H, nx)) return false;
         out.push_back(Item1{inst, (int32_t)g0, (int32_t)x0, (int32_t)w});
         x0 += w;
     }
@@ -333,7 +341,8 @@ static shll_status engine_build_items(Engine* E) {
         const Inst& in = E->h_inst[i];
         E->inst_first_item[r] = E->dim == 1 ? (int64_t)it1.size() : (int64_t)it2.size();
         if (E->dim == 1) {
-            if (!tile1d((int32_t)i, in.nx, E->order, E->L.C, E->L.k, it1))
+            if (!tile1d((int32_t)i, in.nx, E->order, E->L.C, E->L.wpb / step1d_items_per_block(E->L.C, E->L.wpb),
+                        E->L.k, it1))
                 return fail(SHLL_EINVAL, "launch shape C=%d k=%d leaves no owned cells (order %d)", E->L.C, E->L.k,
                             E->order);
         } else {
@@ -407,6 +416,11 @@ static StepArgs make_args(const Engine* E, int64_t j, int parity, int64_t n_item
     a.jofs = 0;
     a.jmax = 0;
     return a;
+}
+
+// work items per block of a launch
+static int items_per_block(const Engine* E) {
+    return E->dim == 1 ? step1d_items_per_block(E->L.C, E->L.wpb) : E->L.wpb;
 }
 
 static hipError_t launch_one(const Engine* E, const StepArgs& a, int64_t n_items) {
@@ -489,7 +503,7 @@ static shll_status batch_graph(Engine* E, int parity, int64_t blocks, int g, hip
         a.active = E->d_active;
         a.jofs = q;
         a.jmax = (int32_t)E->jmax;
-        e = launch_one(E, a, blocks * E->L.wpb);
+        e = launch_one(E, a, blocks * items_per_block(E));
     }
     if (e == hipSuccess) e = launch_ctl(E->stream, E->d_ctl, g, 1);
     hipError_t e2 = hipStreamEndCapture(E->stream, &graph);
@@ -511,7 +525,7 @@ static shll_status batch_launches(Engine* E, int64_t nlaunch) {
     int64_t j = E->launches_done;
     const int64_t jend = std::min<int64_t>(E->jmax, j + nlaunch);
     const int g = E->L.graph > 0 ? std::max(2, (E->L.graph / k) & ~1) : 0;
-    const int wpb = E->L.wpb;
+    const int ipb = items_per_block(E);
     int64_t launches = 0;
     bool ctl_set = false;
     if (E->timing) HIPCHECK(hipEventRecord(E->ev0, E->stream));
@@ -523,7 +537,7 @@ static shll_status batch_launches(Engine* E, int64_t nlaunch) {
             }
             // the chunk's first launch has the most active items; round its grid up to a
             // power of two so few graphs cover the whole run
-            const int64_t need = (active_items(E, j) + wpb - 1) / wpb;
+            const int64_t need = (active_items(E, j) + ipb - 1) / ipb;
             int64_t blocks = 1;
             while (blocks < need) blocks <<= 1;
             hipGraphExec_t exec = nullptr;

@@ -48,6 +48,25 @@ __device__ __forceinline__ double from_right(double x) {
     return __hiloint2double(dpp_from_right(hi), dpp_from_right(lo));
 }
 
+// from_left_or(x, e) / from_right_or(x, e): as from_left / from_right, except that lane 0
+// (lane 63) takes e, a value the lane brings from outside the wavefront
+__device__ __forceinline__ float from_left_or(float x, float e) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(e), __float_as_int(x), 0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float from_right_or(float x, float e) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(e), __float_as_int(x), 0x130, 0xf, 0xf, false));
+}
+__device__ __forceinline__ double from_left_or(double x, double e) {
+    const int lo = __builtin_amdgcn_update_dpp(__double2loint(e), __double2loint(x), 0x138, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(__double2hiint(e), __double2hiint(x), 0x138, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double from_right_or(double x, double e) {
+    const int lo = __builtin_amdgcn_update_dpp(__double2loint(e), __double2loint(x), 0x130, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(__double2hiint(e), __double2hiint(x), 0x130, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+
 // wave-uniform value (moves it to a scalar register)
 __device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
 

@@ -10,6 +10,9 @@
 // so a launch of k steps reads the state from HBM once and writes it once; the
 // segment's outer order*k cells on each side are recomputed halo (temporal
 // blocking without LDS and without barriers: wavefronts never wait on each other).
+// With 4 wavefronts per block and at least 2 cells per lane the block's wavefronts
+// instead hold consecutive segments of one item and swap their edge fluxes through LDS
+// once a step (one barrier), so only the block's ends carry halo.
 //
 // References: base_shll.c:87-177 (first order, reflective), 2nd_order_base_shll.c
 // :203-459 (second order, outflow; its x sweep on the 3-variable system is the 1D
@@ -51,6 +54,101 @@ __device__ __forceinline__ void make_window(const RT (&q)[C], RT (&W)[C + 4]) {
     }
 }
 
+// Cooperative blocks: the XW wavefronts of a block hold consecutive segments of one item.
+// Each step they swap the split fluxes of their two edge cells on each side (every
+// variable, F+ and F-) through LDS, so a segment edge inside the block sees its true
+// neighbours and only the block's two ends carry recomputed halo.  XW = 1: independent
+// wavefronts.
+//
+// LDS, per step parity and wavefront, three slots of 12 values (variable v: F+ of two
+// cells, then F- of the same two, index 4v + {0, 1, 2, 3}): slot 0 the first two cells (lane 0), slot 1 the
+// last two (lane 63), slot 2 a sink for the other pair of each edge lane, so the writes
+// need no per-lane choice of values.  The area is double-buffered by step parity, so one
+// barrier per step orders the writes before the reads and the reads before the next
+// writes to the same buffer.
+template <class RT>
+struct CoopSlots {
+    static constexpr int NV = 16 / sizeof(RT);  // values per 16-byte access
+    typedef RT V __attribute__((ext_vector_type(16 / sizeof(RT))));
+    static constexpr int SLOT = 12, WAVE = 3 * SLOT;
+};
+
+template <class RT, int XW>
+struct Coop {
+    RT* lds;   // [2 parity][XW waves][3 slots][12]
+    int wib;   // wavefront in block
+    int lane;
+};
+
+template <class RT, int NQ>
+__device__ __forceinline__ void slot_store(RT* p, const RT (&q)[NQ]) {
+    using CS = CoopSlots<RT>;
+#pragma unroll
+    for (int i = 0; i < CS::SLOT; i += CS::NV) {
+        typename CS::V w;
+#pragma unroll
+        for (int j = 0; j < CS::NV; ++j) w[j] = q[i + j];
+        *(typename CS::V*)(p + i) = w;
+    }
+}
+
+// publish the split fluxes of the wavefront's first two and last two cells, then wait for
+// the block
+template <class RT, int C, int XW>
+__device__ __forceinline__ void coop_put(const Coop<RT, XW>& x, int parity, const RT (&fp)[3][C],
+                                         const RT (&fm)[3][C]) {
+    using CS = CoopSlots<RT>;
+    RT* w = x.lds + (parity * XW + x.wib) * CS::WAVE;
+    if (x.lane == 0 || x.lane == 63) {
+        RT first[12], last[12];
+#pragma unroll
+        for (int v = 0; v < 3; ++v) {
+            first[4 * v] = fp[v][0];
+            first[4 * v + 1] = fp[v][1];
+            first[4 * v + 2] = fm[v][0];
+            first[4 * v + 3] = fm[v][1];
+            last[4 * v] = fp[v][C - 2];
+            last[4 * v + 1] = fp[v][C - 1];
+            last[4 * v + 2] = fm[v][C - 2];
+            last[4 * v + 3] = fm[v][C - 1];
+        }
+        const bool l0 = x.lane == 0;
+        slot_store(w + (l0 ? 0 : 2) * CS::SLOT, first);
+        slot_store(w + (l0 ? 2 : 1) * CS::SLOT, last);
+    }
+    __syncthreads();
+}
+
+// what lane 0 takes for its left neighbour cells (the previous wavefront's last two) and
+// lane 63 for its right ones (the next wavefront's first two), variable v: e = F+ of the
+// two cells, then F-; the block's end lanes read their own wavefront's values, which only
+// reach the recomputed halo
+template <class RT, int XW>
+__device__ __forceinline__ void coop_get(const Coop<RT, XW>& x, int parity, int v, RT (&e)[4]) {
+    using CS = CoopSlots<RT>;
+    const int src = x.lane == 0 ? (x.wib > 0 ? x.wib - 1 : 0) * CS::WAVE + CS::SLOT
+                                : (x.wib + 1 < XW ? x.wib + 1 : x.wib) * CS::WAVE;
+    const RT* p = x.lds + parity * XW * CS::WAVE + src + 4 * v;
+#pragma unroll
+    for (int i = 0; i < 4; i += CS::NV) {
+        const typename CS::V w = *(const typename CS::V*)(p + i);
+#pragma unroll
+        for (int j = 0; j < CS::NV; ++j) e[i + j] = w[j];
+    }
+}
+
+// make_window with lane 0's two left cells and lane 63's two right cells taken from e0, e1
+template <class RT, int C>
+__device__ __forceinline__ void make_window_or(const RT (&q)[C], RT (&W)[C + 4], RT e0, RT e1) {
+    static_assert(C >= 2, "cooperative blocks hold at least 2 cells per lane");
+#pragma unroll
+    for (int m = 0; m < C; ++m) W[m + 2] = q[m];
+    W[0] = from_left_or(q[C - 2], e0);
+    W[1] = from_left_or(q[C - 1], e1);
+    W[C + 2] = from_right_or(q[0], e0);
+    W[C + 3] = from_right_or(q[1], e1);
+}
+
 // Walls as ghost cells: before each step the state of cell -1 becomes that of cell 0
 // and the state of cell n that of cell n-1 -- copied for outflow walls, mirrored
 // (momentum negated) for reflective ones.  Every operation of the split flux is odd
@@ -89,8 +187,9 @@ __device__ __forceinline__ void ghost_cells(RT (&q)[3][C], int gl, int n, bool r
 // ---------------------------------------------------------------------------
 // F64 / REF32: the reference association, cell by cell
 // ---------------------------------------------------------------------------
-template <int PREC, int ORDER, int C, int WM, int LIM>
-__device__ __forceinline__ void step_ref(typename Real<PREC>::T (&u)[3][C], const Inst& in, int gl) {
+template <int PREC, int ORDER, int C, int WM, int LIM, int XW>
+__device__ __forceinline__ void step_ref(typename Real<PREC>::T (&u)[3][C], const Inst& in, int gl,
+                                         const Coop<typename Real<PREC>::T, XW>& x, int parity) {
     using RT = typename Real<PREC>::T;
     using A = RefArith<RT>;
     const RT lam = PREC == PREC_F64 ? (RT)in.k.d_lamx : (RT)in.k.f_lamx;
@@ -109,11 +208,19 @@ __device__ __forceinline__ void step_ref(typename Real<PREC>::T (&u)[3][C], cons
             fm[v][m] = q[v];
         }
     }
+    if constexpr (XW > 1) coop_put<RT, C, XW>(x, parity, fp, fm);
 #pragma unroll
     for (int v = 0; v < 3; ++v) {
         RT FP[C + 4], FM[C + 4];
-        make_window<RT, C>(fp[v], FP);
-        make_window<RT, C>(fm[v], FM);
+        if constexpr (XW > 1) {
+            RT e[4];
+            coop_get<RT, XW>(x, parity, v, e);
+            make_window_or<RT, C>(fp[v], FP, e[0], e[1]);
+            make_window_or<RT, C>(fm[v], FM, e[2], e[3]);
+        } else {
+            make_window<RT, C>(fp[v], FP);
+            make_window<RT, C>(fm[v], FM);
+        }
 #pragma unroll
         for (int m = 0; m < C; ++m) {
             const int g = gl + m;
@@ -138,8 +245,9 @@ __device__ __forceinline__ void step_ref(typename Real<PREC>::T (&u)[3][C], cons
 // ---------------------------------------------------------------------------
 // KAHAN: the face terms t[m] - t[m+1] enter each cell by one Kahan step (lo holds
 // the part of the value not yet in hi) instead of the exact TwoSum chain
-template <int ORDER, int C, int WM, bool KAHAN, int LIM>
-__device__ __forceinline__ void step_c32(float (&hi)[3][C], float (&lo)[3][C], const Inst& in, int gl) {
+template <int ORDER, int C, int WM, bool KAHAN, int LIM, int XW>
+__device__ __forceinline__ void step_c32(float (&hi)[3][C], float (&lo)[3][C], const Inst& in, int gl,
+                                         const Coop<float, XW>& x, int parity) {
     using A = C32Arith;
     const float lam = in.k.c_lamx, hlam = 0.5f * lam, th = in.k.c_theta;
     const int n = in.nx;
@@ -155,11 +263,19 @@ __device__ __forceinline__ void step_c32(float (&hi)[3][C], float (&lo)[3][C], c
             fm[v][m] = q[v];
         }
     }
+    if constexpr (XW > 1) coop_put<float, C, XW>(x, parity, fp, fm);
 #pragma unroll
     for (int v = 0; v < 3; ++v) {
         float FP[C + 4], FM[C + 4], t[C + 1];
-        make_window<float, C>(fp[v], FP);
-        make_window<float, C>(fm[v], FM);
+        if constexpr (XW > 1) {
+            float e[4];
+            coop_get<float, XW>(x, parity, v, e);
+            make_window_or<float, C>(fp[v], FP, e[0], e[1]);
+            make_window_or<float, C>(fm[v], FM, e[2], e[3]);
+        } else {
+            make_window<float, C>(fp[v], FP);
+            make_window<float, C>(fm[v], FM);
+        }
 #pragma unroll
         for (int f = 0; f <= C; ++f) {
             const int gf = gl + f;  // face gf lies between cells gf-1 and gf
@@ -195,10 +311,10 @@ __device__ __forceinline__ void step_c32(float (&hi)[3][C], float (&lo)[3][C], c
 // wall mode gets its own copy of loop and store, so the loop-carried state never
 // has to be moved between the copies' registers.
 // ---------------------------------------------------------------------------
-template <int ORDER, int C, int WM, bool KAHAN, int LIM>
+template <int ORDER, int C, int WM, bool KAHAN, int LIM, int XW>
 __device__ __forceinline__ void advance_c32(float (&hi)[3][C], float (&lo)[3][C], const Inst& in, const Item1& it,
-                                            int gl, int ns, const StepArgs& a) {
-    for (int s = 0; s < ns; ++s) step_c32<ORDER, C, WM, KAHAN, LIM>(hi, lo, in, gl);
+                                            int gl, int ns, const StepArgs& a, const Coop<float, XW>& x) {
+    for (int s = 0; s < ns; ++s) step_c32<ORDER, C, WM, KAHAN, LIM, XW>(hi, lo, in, gl, x, s & 1);
     float* __restrict__ oh = (float*)a.out0 + in.off;
     float* __restrict__ ol = (float*)a.out1 + in.off;
     const int64_t P = a.plane;
@@ -215,11 +331,12 @@ __device__ __forceinline__ void advance_c32(float (&hi)[3][C], float (&lo)[3][C]
     }
 }
 
-template <int PREC, int ORDER, int C, int WM, int LIM>
+template <int PREC, int ORDER, int C, int WM, int LIM, int XW>
 __device__ __forceinline__ void advance_ref(typename Real<PREC>::T (&u)[3][C], const Inst& in, const Item1& it,
-                                            int gl, int ns, const StepArgs& a) {
+                                            int gl, int ns, const StepArgs& a,
+                                            const Coop<typename Real<PREC>::T, XW>& x) {
     using RT = typename Real<PREC>::T;
-    for (int s = 0; s < ns; ++s) step_ref<PREC, ORDER, C, WM, LIM>(u, in, gl);
+    for (int s = 0; s < ns; ++s) step_ref<PREC, ORDER, C, WM, LIM, XW>(u, in, gl, x, s & 1);
     RT* __restrict__ ou = (RT*)a.out0 + in.off;
     const int64_t P = a.plane;
 #pragma unroll
@@ -238,20 +355,29 @@ __device__ __forceinline__ void advance_ref(typename Real<PREC>::T (&u)[3][C], c
 template <int PREC, int ORDER, int C, int WPB, int LIM>
 __global__ __launch_bounds__(64 * WPB) void k_step1d(StepArgs a) {
     using RT = typename Real<PREC>::T;
-    const int wave = (int)blockIdx.x * WPB + uniform((int)threadIdx.x >> 6);
+    // blocks of several wavefronts with at least 2 cells per lane are cooperative: one item
+    // per block, one segment of 64*C cells per wavefront
+    constexpr int XW = WPB > 1 && C >= 2 ? WPB : 1;
+    __shared__ __align__(16) RT xlds[XW > 1 ? 2 * XW * CoopSlots<RT>::WAVE : 1];
+    const int wib = uniform((int)threadIdx.x >> 6);
+    const int item = XW > 1 ? (int)blockIdx.x : (int)blockIdx.x * WPB + wib;
     const LaunchView lv = launch_view(a);
-    if (wave >= lv.n_items) return;
+    // every wavefront of a cooperative block has the same item and step count, so all of
+    // them return here together or reach the same number of barriers
+    if (item >= lv.n_items) return;
     const int lane = (int)threadIdx.x & 63;
-    const Item1 it = ((const Item1*)a.items)[wave];
+    const Item1 it = ((const Item1*)a.items)[item];
     const Inst in = a.inst[it.inst];
     const int64_t left = in.nsteps - lv.step_base;
     const int ns = left < (int64_t)a.ksteps ? (int)left : a.ksteps;
     if (ns <= 0) return;
     const int n = in.nx;
-    const int gl = it.g0 + lane * C;
+    const int gw = it.g0 + (XW > 1 ? wib * 64 * C : 0);  // this wavefront's first cell
+    const int gl = gw + lane * C;
     const int64_t P = a.plane;
+    const Coop<RT, XW> x{xlds, wib, lane};
     // wall rules can only apply if the segment reaches within 3 cells of an end
-    const bool walls = it.g0 <= 2 || it.g0 + 64 * C >= n - 3;
+    const bool walls = gw <= 2 || gw + 64 * C >= n - 3;
     const int wm = !walls ? WM_NONE : (ORDER == 2 && in.bc == BC_REFLECT ? WM_RULES : WM_GHOST);
     constexpr int WR = ORDER == 2 ? WM_RULES : WM_GHOST;
 
@@ -270,9 +396,9 @@ __global__ __launch_bounds__(64 * WPB) void k_step1d(StepArgs a) {
                 lo[v][m] = ok ? il[v * P + g] : 0.0f;
             }
         }
-        if (wm == WM_NONE) advance_c32<ORDER, C, WM_NONE, K, LIM>(hi, lo, in, it, gl, ns, a);
-        else if (ORDER == 1 || wm == WM_GHOST) advance_c32<ORDER, C, WM_GHOST, K, LIM>(hi, lo, in, it, gl, ns, a);
-        else advance_c32<ORDER, C, WR, K, LIM>(hi, lo, in, it, gl, ns, a);
+        if (wm == WM_NONE) advance_c32<ORDER, C, WM_NONE, K, LIM, XW>(hi, lo, in, it, gl, ns, a, x);
+        else if (ORDER == 1 || wm == WM_GHOST) advance_c32<ORDER, C, WM_GHOST, K, LIM, XW>(hi, lo, in, it, gl, ns, a, x);
+        else advance_c32<ORDER, C, WR, K, LIM, XW>(hi, lo, in, it, gl, ns, a, x);
     } else {
         const RT* __restrict__ iu = (const RT*)a.in0 + in.off;
         RT u[3][C];
@@ -283,9 +409,9 @@ __global__ __launch_bounds__(64 * WPB) void k_step1d(StepArgs a) {
 #pragma unroll
             for (int v = 0; v < 3; ++v) u[v][m] = ok ? iu[v * P + g] : (RT)(v == 0 ? 1.0 : (v == 1 ? 0.0 : 2.5));
         }
-        if (wm == WM_NONE) advance_ref<PREC, ORDER, C, WM_NONE, LIM>(u, in, it, gl, ns, a);
-        else if (ORDER == 1 || wm == WM_GHOST) advance_ref<PREC, ORDER, C, WM_GHOST, LIM>(u, in, it, gl, ns, a);
-        else advance_ref<PREC, ORDER, C, WR, LIM>(u, in, it, gl, ns, a);
+        if (wm == WM_NONE) advance_ref<PREC, ORDER, C, WM_NONE, LIM, XW>(u, in, it, gl, ns, a, x);
+        else if (ORDER == 1 || wm == WM_GHOST) advance_ref<PREC, ORDER, C, WM_GHOST, LIM, XW>(u, in, it, gl, ns, a, x);
+        else advance_ref<PREC, ORDER, C, WR, LIM, XW>(u, in, it, gl, ns, a, x);
     }
 }
 
@@ -322,9 +448,13 @@ static hipError_t launch_p(int order, int lim, int C, int wpb, int nblocks, hipS
     return hipErrorInvalidValue;
 }
 
+// items (segments) per block: one for cooperative blocks (several waves, C >= 2), else one per wave
+int step1d_items_per_block(int C, int wpb) { return wpb > 1 && C >= 2 ? 1 : wpb; }
+
 hipError_t launch_step1d(int prec, int order, int lim, int C, int wpb, int n_items, hipStream_t s,
                          const StepArgs& a) {
-    const int nblocks = (n_items + wpb - 1) / wpb;
+    const int ipb = step1d_items_per_block(C, wpb);
+    const int nblocks = (n_items + ipb - 1) / ipb;
     if (nblocks <= 0) return hipSuccess;
     switch (prec) {
     case PREC_F64: return launch_p<PREC_F64>(order, lim, C, wpb, nblocks, s, a);

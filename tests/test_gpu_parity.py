@@ -68,7 +68,7 @@ def test_1d_sod_full_run(orc, prec, order):
 
 
 @pytest.mark.parametrize("launch", [(1, 1, 1, 0), (1, 8, 4, 64), (2, 3, 1, 0), (4, 16, 4, 128), (8, 5, 4, 20),
-                                    (2, 24, 1, 0), (8, 1, 1, 8), (8, 60, 1, 0)])
+                                    (2, 24, 1, 0), (8, 1, 1, 8), (8, 60, 1, 0), (2, 2, 4, 0)])
 @pytest.mark.parametrize("prec", [S.F64, S.C32, S.C32K])
 def test_1d_launch_shapes_are_bitwise_identical(orc, launch, prec):
     """Every (cells/lane, steps/launch, waves/block, graph) gives the oracle's bits."""
@@ -77,6 +77,21 @@ def test_1d_launch_shapes_are_bitwise_identical(orc, launch, prec):
     ic = icgen.random_1d(n, 5, block=37)
     banner("1d shapes", launch, prec)
     assert_same(pb, gpu_run(pb, ic, 150, launch), oracle_run(orc, pb, ic, 150), "shape %s" % (launch,))
+
+
+@pytest.mark.parametrize("launch,sizes", [((2, 3, 4, 0), (127, 255, 602, 858, 1082)),
+                                          ((4, 7, 4, 0), (255, 767, 1234, 1746)),
+                                          ((8, 16, 4, 0), (511, 1535, 2464, 3488))])
+@pytest.mark.parametrize("bc", [0, 1])
+def test_1d_cooperative_blocks_wall_at_a_wavefront_edge(orc, launch, sizes, bc):
+    """Blocks of 4 wavefronts with 2+ cells per lane share edge fluxes through LDS; the
+    tiles of these sizes would put ghost cell n on a wavefront's first cell, which the
+    tiler moves one cell on.  Both walls, second order, bitwise."""
+    for n in sizes:
+        pb = S.problem(1, 2, n, precision=S.C32K, bc=bc)
+        ic = icgen.random_1d(n, 60 + bc, block=7)
+        banner("1d coop", launch, n, bc)
+        assert_same(pb, gpu_run(pb, ic, 40, launch), oracle_run(orc, pb, ic, 40), "coop %s n=%d" % (launch, n))
 
 
 def test_1d_impossible_launch_shape_is_rejected():
