@@ -107,6 +107,39 @@ def calibration_copy(nbytes):
     log("calibration copy: %d bytes device-to-device" % nbytes)
 
 
+def copy_ceiling(nbytes=1 << 30, reps=10):
+    """Measured HBM ceiling: median rate of reps device-to-device copies of nbytes (the HIP
+    runtime's copy kernel; each moves 2 x nbytes), timed with HIP events, in GB/s."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) or hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)):
+        raise RuntimeError("hipMalloc failed for the copy ceiling")
+    e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
+    hip.hipEventCreate(ctypes.byref(e0))
+    hip.hipEventCreate(ctypes.byref(e1))
+    hip.hipMemset(a, 1, ctypes.c_size_t(nbytes))
+    rates = []
+    for r in range(reps + 1):
+        hip.hipEventRecord(e0, None)
+        rc = hip.hipMemcpyAsync(b, a, ctypes.c_size_t(nbytes), 3, None)  # hipMemcpyDeviceToDevice
+        hip.hipEventRecord(e1, None)
+        hip.hipEventSynchronize(e1)
+        ms = ctypes.c_float()
+        hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1)
+        if rc:
+            break
+        if r > 0:  # the first copy warms up
+            rates.append(2 * nbytes / (ms.value * 1e-3) / 1e9)
+    hip.hipEventDestroy(e0)
+    hip.hipEventDestroy(e1)
+    hip.hipFree(a)
+    hip.hipFree(b)
+    if rc or not rates:
+        raise RuntimeError("copy ceiling: hipMemcpyAsync failed (%d)" % rc)
+    return sorted(rates)[len(rates) // 2]
+
+
 def run_batch(S, pbs, prims, steps, launch, K, W, progress=0, trial=2048):
     b = S.Batch(pbs, prims, steps)
     if launch:
@@ -449,6 +482,7 @@ def main_domains(S, args):
             "kernel": "shll::k_step2d_tb<%d, %d, %d> (FP32-compensated, first order, %d steps per launch)" % (
                 form, lau[1], lau[2], lau[1]) if lau[1] > 1 else "shll::k_step2d<%d, 1, %d, 0>" % (form, lau[2]),
             "algorithmic_bytes_per_cell_update": B, "avg_launch_ms": avg_ms,
+            "measured_copy_GBps": copy_ceiling(),
             "note": "achieved = 64 B x cell-updates / HIP-event time of the launches (k steps per launch read "
                     "and write the state once). traffic = calibrated rocprofv3 FETCH_SIZE+WRITE_SIZE per "
                     "cell-update (profiles/k_step2d_tb_%s_domains64_hbm_traffic.md) x cell-updates per launch"
@@ -561,6 +595,7 @@ def main():
                 form, lau[0], lau[2], "Kahan" if form == S.C32K else "TwoSum", lau[0], lau[2]),
             "algorithmic_bytes_per_cell_update": B,
             "avg_launch_ms": avg_ms,
+            "measured_copy_GBps": copy_ceiling(),
             "note": "achieved = 48 B x cell-updates / HIP-event time of the launches; temporal blocking "
                     "(k steps per launch from registers) lets it exceed the HBM peak. traffic = HBM bytes per "
                     "launch of the timed run: calibrated rocprofv3 FETCH_SIZE+WRITE_SIZE per cell-update "
