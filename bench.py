@@ -274,7 +274,7 @@ def extra_single(S, cpu=True):
     for name, prec in (("f64", S.F64), ("c32", S.C32), ("c32k", S.C32K)):
         pb = S.problem(1, 2, n1, precision=prec)
         s = S.Solver(pb, ic)
-        s.autotune(2000)
+        s.autotune(8000)
         shape = s.get_launch()
         s.step(4000)
         s.close()

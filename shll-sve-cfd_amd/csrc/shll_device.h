@@ -48,6 +48,26 @@ __device__ __forceinline__ double from_right(double x) {
     return __hiloint2double(dpp_from_right(hi), dpp_from_right(lo));
 }
 
+// from_left_z(x) / from_right_z(x): as from_left / from_right, except that lane 0 (lane 63)
+// reads 0 -- for values whose edge lanes are don't-cares; without an "old" operand the
+// shift can be folded into the instruction that uses it
+__device__ __forceinline__ float from_left_z(float x) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x138, 0xf, 0xf, true));
+}
+__device__ __forceinline__ float from_right_z(float x) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x130, 0xf, 0xf, true));
+}
+__device__ __forceinline__ double from_left_z(double x) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), 0x138, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), 0x138, 0xf, 0xf, true);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double from_right_z(double x) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), 0x130, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), 0x130, 0xf, 0xf, true);
+    return __hiloint2double(hi, lo);
+}
+
 // from_left_or(x, e) / from_right_or(x, e): as from_left / from_right, except that lane 0
 // (lane 63) takes e, a value the lane brings from outside the wavefront
 __device__ __forceinline__ float from_left_or(float x, float e) {

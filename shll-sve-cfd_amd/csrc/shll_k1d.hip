@@ -36,12 +36,23 @@ __device__ __forceinline__ RT slope_at(const RT (&W)[NW], int p, int gg, int n, 
     return (gg <= 0 || gg >= n - 1) ? (RT)0 : s;
 }
 
-// window of a per-lane array q[C]: W[p+2] = value at lane position p, p in [-2, C+1]
-template <class RT, int C>
+// window of a per-lane array q[C]: W[p+2] = value at lane position p, p in [-2, C+1].
+// The segment's end lanes get don't-care values for their missing neighbours (recomputed
+// halo): with ZERO, zeros by shifts without an "old" operand, kept as register moves (an
+// empty asm statement stops their folding into the instructions that use them); without,
+// the lane's own values.  Both only change the instruction schedule: the exact TwoSum
+// form measured 7% faster with ZERO, the Kahan form 4% slower.
+template <class RT, int C, bool ZERO>
 __device__ __forceinline__ void make_window(const RT (&q)[C], RT (&W)[C + 4]) {
 #pragma unroll
     for (int m = 0; m < C; ++m) W[m + 2] = q[m];
-    if constexpr (C >= 2) {
+    if constexpr (C >= 2 && ZERO) {
+        W[0] = from_left_z(q[C - 2]);
+        W[1] = from_left_z(q[C - 1]);
+        W[C + 2] = from_right_z(q[0]);
+        W[C + 3] = from_right_z(q[1]);
+        asm volatile("" : "+v"(W[0]), "+v"(W[1]), "+v"(W[C + 2]), "+v"(W[C + 3]));
+    } else if constexpr (C >= 2) {
         W[0] = from_left(q[C - 2]);
         W[1] = from_left(q[C - 1]);
         W[C + 2] = from_right(q[0]);
@@ -218,8 +229,8 @@ __device__ __forceinline__ void step_ref(typename Real<PREC>::T (&u)[3][C], cons
             make_window_or<RT, C>(fp[v], FP, e[0], e[1]);
             make_window_or<RT, C>(fm[v], FM, e[2], e[3]);
         } else {
-            make_window<RT, C>(fp[v], FP);
-            make_window<RT, C>(fm[v], FM);
+            make_window<RT, C, false>(fp[v], FP);
+            make_window<RT, C, false>(fm[v], FM);
         }
 #pragma unroll
         for (int m = 0; m < C; ++m) {
@@ -273,8 +284,8 @@ __device__ __forceinline__ void step_c32(float (&hi)[3][C], float (&lo)[3][C], c
             make_window_or<float, C>(fp[v], FP, e[0], e[1]);
             make_window_or<float, C>(fm[v], FM, e[2], e[3]);
         } else {
-            make_window<float, C>(fp[v], FP);
-            make_window<float, C>(fm[v], FM);
+            make_window<float, C, !KAHAN>(fp[v], FP);
+            make_window<float, C, !KAHAN>(fm[v], FM);
         }
 #pragma unroll
         for (int f = 0; f <= C; ++f) {
