@@ -319,7 +319,7 @@ __device__ __forceinline__ void stage2(Held2<PREC>& h, typename Real<PREC>::T (&
     for (int v = 0; v < 4; ++v) {
         const RT sx = (RT)sgn_x(v), sy = (RT)sgn_y(v);
         const RT fpc = xp[v], fmc = xm[v], hpc = yp[v], hmc = ym[v];
-        const RT hpl = from_left(hpc), hmr = from_right(hmc);
+        const RT hpl = from_left_z(hpc), hmr = from_right_z(hmc);
         if constexpr (!CMP) {
             h.tl[v] = mode == ST_TAKE_FIRST ? (refl ? sx * fmc : fpc) : h.fp[v];
             const RT Bv = YW ? (w.lo_refl ? sy * hmc : (w.lo ? hpc : hpl)) : hpl;
@@ -331,7 +331,7 @@ __device__ __forceinline__ void stage2(Held2<PREC>& h, typename Real<PREC>::T (&
             // wall faces: (mirrored or own) flux + own flux; the sum commutes exactly
             const RT Gb = (YW ? (w.lo_refl ? sy * hmc : (w.lo ? hpc : hpl)) : hpl) + hmc;
             const RT tb = lamy * Gb;
-            RT tt = from_right(tb);
+            RT tt = from_right_z(tb);
             if constexpr (YW) tt = w.hi ? lamy * (hpc + (w.hi_refl ? sy * hpc : hmc)) : tt;
             if constexpr (PREC == PREC_C32K) {
                 h.yb[v] = tb - tt;
