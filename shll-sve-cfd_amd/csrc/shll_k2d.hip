@@ -126,8 +126,8 @@ __global__ __launch_bounds__(64 * WPB) void k_step2d(StepArgs a) {
             const RT sx = (RT)sgn_x(v), sy = (RT)sgn_y(v);
             const RT fpc = xp[ORDER][v], fmc = xm[ORDER][v];
             const RT hpc = yp[0][v], hmc = ym[0][v];
-            const RT hpl = from_left(hpc), hml = from_left(hmc);
-            const RT hpr = from_right(hpc), hmr = from_right(hmc);
+            const RT hpl = from_left_z(hpc), hml = from_left_z(hmc);
+            const RT hpr = from_right_z(hpc), hmr = from_right_z(hmc);
             // slopes (second order); zero on boundary rows / columns
             RT dfp = 0, dfm = 0, dfpl = 0, dfmr = 0, dhp = 0, dhm = 0, dhpl = 0, dhmr = 0;
             if constexpr (ORDER == 2) {
@@ -139,8 +139,8 @@ __global__ __launch_bounds__(64 * WPB) void k_step2d(StepArgs a) {
                                                        : limited_slope<A, LIM>(xm[2][v], xm[3][v], xm[4][v], th);
                 dhp = (j <= 0 || j >= ny - 1) ? (RT)0 : limited_slope<A, LIM>(hpl, hpc, hpr, th);
                 dhm = (j <= 0 || j >= ny - 1) ? (RT)0 : limited_slope<A, LIM>(hml, hmc, hmr, th);
-                dhpl = from_left(dhp);
-                dhmr = from_right(dhm);
+                dhpl = from_left_z(dhp);
+                dhmr = from_right_z(dhm);
             }
             if constexpr (!CMP) {
                 // reference association
@@ -179,7 +179,7 @@ __global__ __launch_bounds__(64 * WPB) void k_step2d(StepArgs a) {
                     tr = __builtin_fmaf(0.5f * lamx, dr, tr);
                     tb = __builtin_fmaf(0.5f * lamy, db, tb);
                 }
-                RT tt = from_right(tb);
+                RT tt = from_right_z(tb);
                 if constexpr (ORDER == 2)
                     tt = j == ny - 1 ? __builtin_fmaf(0.5f * lamy, (RT)0, lamy * (refl ? hpc + sy * hpc : hpc + hmc)) : tt;
                 else
@@ -533,22 +533,22 @@ __device__ __forceinline__ void stage2o2(Held2o2<PREC>& h, float (&iou)[4], floa
         for (int v = 0; v < 4; ++v) {
             const float sy = (float)sgn_y(v);
             const float hpc = yp[v], hmc = ym[v];
-            const float hpl = from_left(hpc), hpr = from_right(hpc);
-            const float hml = from_left(hmc), hmr = from_right(hmc);
+            const float hpl = from_left_z(hpc), hpr = from_right_z(hpc);
+            const float hml = from_left_z(hmc), hmr = from_right_z(hmc);
             float dhp = limited_slope<A, LIM>(hpl, hpc, hpr, th);
             float dhm = limited_slope<A, LIM>(hml, hmc, hmr, th);
             if constexpr (YW) {
                 dhp = yedge ? 0.0f : dhp;
                 dhm = yedge ? 0.0f : dhm;
             }
-            const float dhpl = from_left(dhp);
+            const float dhpl = from_left_z(dhp);
             float Gb = hpl + hmc, db = dhpl - dhm;
             if constexpr (YW) {
                 Gb = w.lo ? (w.lo_refl ? hmc + sy * hmc : hpc + hmc) : Gb;
                 db = w.lo ? 0.0f : db;
             }
             const float tb = __builtin_fmaf(hly, db, lamy * Gb);
-            float tt = from_right(tb);
+            float tt = from_right_z(tb);
             if constexpr (YW)
                 tt = w.hi ? __builtin_fmaf(hly, 0.0f, lamy * (w.hi_refl ? hpc + sy * hpc : hpc + hmc)) : tt;
             if constexpr (PREC == PREC_C32K) {
