@@ -549,7 +549,7 @@ def main():
     judge = None if args.no_accuracy else make_judge(S, pbs, steps, prims, K)
     errs = tube_errors(S, b, pbs, judge)
     b.close()
-    alt = None
+    alt = f64b = None
     if not args.no_extra:
         # the same batch with the other compensated update, its own tuned shape
         pbo = [S.problem(1, 2, p.nx, precision=other) for p in pbs]
@@ -559,6 +559,15 @@ def main():
                "launch_shape": list(bo.get_launch()), "accuracy": tube_errors(S, bo, pbo, judge)}
         log("other form: %.3e cell-updates/s" % (upd / wo))
         bo.close()
+        # the same batch in FP64 (state and fluxes): the precision that meets any error bar
+        # against the FP64 oracle (it is bitwise that oracle), at its own tuned shape
+        pbf = [S.problem(1, 2, p.nx, precision=S.F64) for p in pbs]
+        bf, wf, af, lf, _ = run_batch(S, pbf, prims, steps, None, K, args.warmup, args.progress)
+        f64b = {"form": "FP64 state and fluxes, reference association (bitwise the FP64 CPU oracle)",
+                "cell_updates_per_s": upd / wf, "avg_launch_ms": af, "launches": lf,
+                "launch_shape": list(bf.get_launch())}
+        log("FP64 batch: %.3e cell-updates/s" % (upd / wf))
+        bf.close()
     if judge is not None:
         judge[0].close()
     tpu = traffic_per_update(lau, args.form)
@@ -609,7 +618,7 @@ def main():
     else:
         result["cpu_baseline"] = None
     if not args.no_extra:
-        result["extra"] = {"c3_other_update": alt}
+        result["extra"] = {"c3_other_update": alt, "c3_f64_batch": f64b}
         result["extra"].update(extra_single(S, cpu=not args.no_cpu))
         if not args.no_cpu:
             result["extra"]["cpu_reference_solvers"] = reference_cpu()

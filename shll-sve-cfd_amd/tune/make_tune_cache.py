@@ -21,10 +21,10 @@ def main():
     os.environ["SHLL_TUNE_CACHE"] = out
     import shll_amd as S
 
-    # batched FP32-compensated second-order tubes (the benchmark batch); the trial
+    # batched second-order tubes (the benchmark batch: both compensated forms and FP64); the trial
     # window (2048 steps, every tube of >= 1280 cells still running) weights the tube
     # sizes roughly as the whole run does
-    for prec in (S.C32, S.C32K):
+    for prec in (S.C32, S.C32K, S.F64):
         pbs, prims, steps = S.tube_batch(20261202, 4096, precision=prec)
         b = S.Batch(pbs, prims, steps)
         b.autotune(2048)
