@@ -228,12 +228,13 @@ struct C32Arith {
         T = __builtin_fmaf(-0.5f, __builtin_fmaf(vx, vx, vy * vy), e) * rcv;
         a = sqrt_fast(gR * T);
     }
-    // z1 = (M+1)/2, z2 = (a/2)(1-M^2), z3 = (M-1)/2
+    // z1 = (M+1)/2, z2 = (a/2)(1-M^2), z3 = (M-1)/2.  z1 and z3 are RN(M/2 +- 1/2): M/2 is
+    // exact (or, below 2^-125, off by less than 2^-150, which the sum rounds away), so one
+    // fused multiply-add gives the bits of the product-then-sum
     __device__ static __forceinline__ void zinv(float M, float a, float& z1, float& z2, float& z3) {
-        const float h = 0.5f * M;
-        z1 = h + 0.5f;
+        z1 = __builtin_fmaf(0.5f, M, 0.5f);
         z2 = (0.5f * a) * __builtin_fmaf(-M, M, 1.0f);
-        z3 = h - 0.5f;
+        z3 = __builtin_fmaf(0.5f, M, -0.5f);
     }
     // minmod as the median of (l, r, 0): the smaller magnitude when the signs agree,
     // zero otherwise -- one v_med3_f32.  Unlike the reference's product test it also
