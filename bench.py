@@ -252,7 +252,7 @@ def extra_single(S, cpu=True):
     """The single-instance configurations, each in full, with errors against FP64 and (cpu)
     the oracle's one-core rate on a few steps of the same problem."""
     out = {}
-    # 1D first order, 1024 cells, t = 0.2 (configs[0]), FP64
+    # 1D first order, 1024 cells, t = 0.2, FP64
     pb = S.problem(1, 1, 1024, precision=S.F64)
     ic = S.ic_tube(1024)
     with S.Solver(pb, ic) as s:
@@ -267,7 +267,7 @@ def extra_single(S, cpu=True):
     out["c0_1d_o1_f64_N1024"] = {"steps": n, "seconds": dt, "cell_updates_per_s": 1024 * n / dt}
     if cpu:
         out["c0_1d_o1_f64_N1024"]["cpu_baseline"] = cpu_rate(pb, ic, n)
-    # 1D second order, 65536 cells, t = 0.2 (configs[1]), FP64 and both FP32-compensated forms
+    # 1D second order, 65536 cells, t = 0.2, FP64 and both FP32-compensated forms
     n1 = 65536
     ic = S.ic_tube(n1)
     sol = {}
@@ -292,7 +292,7 @@ def extra_single(S, cpu=True):
         out["c1_1d_o2_f64_N65536"]["cpu_baseline"] = cpu_rate(S.problem(1, 2, n1), ic, 400)
     for s in sol.values():
         s.close()
-    # 2D first order, 4096^2, 2000 steps (configs[2]), FP64 and FP32-compensated
+    # 2D first order, 4096^2, 2000 steps, FP64 and FP32-compensated
     n2 = 4096
     ic = S.ic_circle(n2, n2)
     sol = {}
@@ -319,7 +319,7 @@ def extra_single(S, cpu=True):
         out["c2_2d_o1_f64_4096sq"]["cpu_baseline"] = cpu_rate(S.problem(2, 1, n2, n2), ic, 2)
     for s in sol.values():
         s.close()
-    # 64 uneven 2D domains, first order, 1000 steps (configs[4]), both FP32-compensated
+    # 64 uneven 2D domains, first order, 1000 steps, both FP32-compensated
     # forms; the FP64 batch of the same domains judges accuracy
     pbs, prims = S.domain_batch(4242, 64, precision=S.C32K)
     cells = sum(p.nx * p.ny for p in pbs)
@@ -520,7 +520,7 @@ def main():
     if not args.autotune:
         os.environ.setdefault("SHLL_TUNE_CACHE", TUNE_CACHE)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
+        print("bench.py: the library runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
         sys.exit(2)
     import shll_amd as S
 

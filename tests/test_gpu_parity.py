@@ -58,7 +58,7 @@ def assert_same(pb, gpu, ora, what):
 @pytest.mark.parametrize("prec", [S.F64, S.REF32, S.C32, S.C32K])
 @pytest.mark.parametrize("order", [1, 2])
 def test_1d_sod_full_run(orc, prec, order):
-    """configs[0] shape: Sod, 1024 cells to t = 0.2 (1639 steps)."""
+    """Sod, 1024 cells to t = 0.2 (1639 steps)."""
     pb = S.problem(1, order, 1024, precision=prec)
     n = S.steps_to_time(pb, 0.2)
     assert n == 1639
@@ -256,7 +256,7 @@ def test_conservation_reflective(prec, tol):
 
 @pytest.mark.parametrize("prec", [S.C32, S.C32K])
 def test_c32_accuracy_against_f64_on_gpu(prec):
-    """FP32-compensated first-order Sod (configs[0] shape) vs the FP64 kernel: L1(rho) <= 1e-6."""
+    """FP32-compensated first-order Sod (1024 cells) vs the FP64 kernel: L1(rho) <= 1e-6."""
     pb64 = S.problem(1, 1, 1024, precision=S.F64)
     pb32 = S.problem(1, 1, 1024, precision=prec)
     ic = S.ic_tube(1024)
