@@ -180,7 +180,8 @@ def test_1d_batch_equals_single_runs(orc, prec, order):
     pbs, prims, steps = S.tube_batch(777, 12, nmin=64, nmax=3000, precision=prec, order=order)
     steps = [max(1, s // 8) for s in steps]
     b = S.Batch(pbs, prims, steps)
-    b.set_launch(2, 8, 4, 32)
+    # cooperative blocks: 2 cells per lane, or the FP64 batch's tuned shape (4 cells, k = 8)
+    b.set_launch(*((4, 8, 4, 32) if prec == S.F64 else (2, 8, 4, 32)))
     banner("1d batch", prec, order)
     b.run()
     for k in range(len(pbs)):
