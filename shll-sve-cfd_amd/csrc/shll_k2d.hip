@@ -400,7 +400,7 @@ __device__ __forceinline__ void pipeline2(const StepArgs& a, const Item2& it, co
     // which gives up its held row when it takes the next one or drains) and drains at r1.
     // Stage K-1 gives row t - K.
     const int tend = r1 + K;
-    for (int t = r0; t < tend; ++t) {
+    auto iter = [&](int t, bool steady) {
         RT U[4], Ul[4];
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
@@ -417,7 +417,6 @@ __device__ __forceinline__ void pipeline2(const StepArgs& a, const Item2& it, co
         load_row(t + D, nu[D - 1], nl[D - 1]);
         // live: U holds a row at the current stage's input level
         bool live = true;
-        const bool steady = ns == K && t - (K - 1) > r0 && t < r1;
         if (steady) {
             // every stage takes an inner row and advances: one straight-line body
 #pragma unroll
@@ -447,7 +446,17 @@ __device__ __forceinline__ void pipeline2(const StepArgs& a, const Item2& it, co
                 if constexpr (CMP) o1p[v * P + o] = Ul[v];
             }
         }
+    };
+    // iterations t in [r0 + K, r1) are steady when every stage advances (ns == K): they run
+    // in a loop of their own, K at a time, apart from the ramp-up and drain iterations
+    int t = r0;
+    if (ns == K && r1 - (r0 + K) >= K) {
+        for (; t < r0 + K; ++t) iter(t, false);
+        for (; t + K <= r1; t += K)
+#pragma unroll
+            for (int q = 0; q < K; ++q) iter(t + q, true);
     }
+    for (; t < tend; ++t) iter(t, ns == K && t - (K - 1) > r0 && t < r1);
 }
 
 template <int PREC, int K, int WPB>
