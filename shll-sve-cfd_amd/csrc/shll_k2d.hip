@@ -691,7 +691,7 @@ __device__ __forceinline__ void pipeline2o2(const StepArgs& a, const Item2& it, 
     // Stage s is at row t - 2s in iteration t: it takes rows [r0, r1) and drains at r1 and
     // r1 + 1, giving row t - 2s - 2 from its third row on.  Stage K-1 gives row t - 2K.
     const int tend = r1 + 2 * K;
-    for (int t = r0; t < tend; ++t) {
+    auto iter = [&](int t, bool steady) {
         float U[4], Ul[4];
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
@@ -707,7 +707,6 @@ __device__ __forceinline__ void pipeline2o2(const StepArgs& a, const Item2& it, 
             }
         load_row(t + D, nu[D - 1], nl[D - 1]);
         bool live = true;  // U holds a row at the current stage's input level
-        const bool steady = ns == K && t - 2 * (K - 1) >= r0 + 2 && t < r1;
         if (steady) {
 #pragma unroll
             for (int s = 0; s < K; ++s)
@@ -738,7 +737,17 @@ __device__ __forceinline__ void pipeline2o2(const StepArgs& a, const Item2& it, 
                 o1p[v * P + o] = Ul[v];
             }
         }
+    };
+    // iterations t in [r0 + 2K, r1) are steady when every stage advances (ns == K): they run
+    // in a loop of their own, apart from the ramp-up and drain iterations
+    int t = r0;
+    if (ns == K && r1 - (r0 + 2 * K) >= 2 * K) {
+        for (; t < r0 + 2 * K; ++t) iter(t, false);
+        for (; t + 2 * K <= r1; t += 2 * K)
+#pragma unroll
+            for (int q = 0; q < 2 * K; ++q) iter(t + q, true);
     }
+    for (; t < tend; ++t) iter(t, ns == K && t - 2 * (K - 1) >= r0 + 2 && t < r1);
 }
 
 template <int PREC, int K, int WPB, int LIM>
