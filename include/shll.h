@@ -47,9 +47,13 @@ typedef enum {
     SHLL_REF32 = 1,  /* the reference's own float arithmetic with its double promotions */
     SHLL_C32 = 2,    /* FP32 fluxes, compensated (hi, lo) FP32 state, conservative face update
                         added exactly (TwoSum): conserved sums exact to ~1e-14 */
-    SHLL_C32K = 3    /* as SHLL_C32 but the face terms enter each cell through one Kahan
+    SHLL_C32K = 3,   /* as SHLL_C32 but the face terms enter each cell through one Kahan
                         update (3x fewer update instructions); same accuracy against FP64,
                         conserved sums to ~1e-10 */
+    SHLL_F64F = 4    /* 1D only: FP64 state and fluxes in SHLL_C32K's arrangement (fused
+                        multiply-adds, one reciprocal of rho, face-term update); not the
+                        reference's association, so within FP64 round-off of SHLL_F64 rather
+                        than bitwise (L1(rho) ~1e-12 on second-order tubes) */
 } shll_precision;
 
 typedef enum {
@@ -112,7 +116,9 @@ shll_status shll_step(shll_ctx *ctx, int64_t nsteps);
  * is checked (SHLL_ENONFINITE). */
 shll_status shll_run_to_time(shll_ctx *ctx, float t_end, int64_t *steps_out);
 
-/* Step count of the reference's float time loop for a problem (no GPU needed). */
+/* Step count of the reference's float time loop for a problem (no GPU needed); -1 for a
+ * NULL or invalid problem, or when the float sum time += DT stops growing before it reaches
+ * t_end (the reference's loop would not end). */
 int64_t shll_steps_to_time(const shll_problem *pb, float t_end);
 
 /* Replaces the arrays Save_Results prints (p0..p3 after Compute_P_from_U,
@@ -120,7 +126,7 @@ int64_t shll_steps_to_time(const shll_problem *pb, float t_end);
 shll_status shll_get_primitives(shll_ctx *ctx, double *out_soa);
 /* Conserved variables (u0..u3) as doubles; C32 gives (double)hi + (double)lo. */
 shll_status shll_get_conserved(shll_ctx *ctx, double *out_soa);
-/* The state in its storage format: F64 nvar*N doubles; REF32 nvar*N floats;
+/* The state in its storage format: F64 and F64F nvar*N doubles; REF32 nvar*N floats;
  * C32 nvar*N hi floats then nvar*N lo floats.  bytes must equal shll_state_bytes(). */
 size_t shll_state_bytes(const shll_ctx *ctx);
 shll_status shll_get_state_raw(shll_ctx *ctx, void *out, size_t bytes);
@@ -187,7 +193,10 @@ shll_status shll_last_timing(const shll_ctx *ctx, double *avg_kernel_ms, int64_t
  * steps over an (instance, tile) work list ordered by remaining work. */
 shll_status shll_batch_init(const shll_problem *items, const double *const *prims, const int64_t *nsteps,
                             int32_t n, shll_batch **out);
-/* Advance every unfinished instance by up to nsteps steps (graph-captured chunks). */
+/* Advance every unfinished instance by exactly min(nsteps, its remaining steps): whole
+ * chunks of steps_per_launch-step launches replay HIP graphs, the last launch of a call
+ * takes the steps that remain.  *max_remaining (may be NULL) receives the steps the longest
+ * instance still has to take. */
 shll_status shll_batch_advance(shll_batch *b, int64_t nsteps, int64_t *max_remaining);
 /* Run every instance to its own step count. */
 shll_status shll_batch_run(shll_batch *b);

@@ -12,7 +12,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 
-F64, REF32, C32, C32K = 0, 1, 2, 3
+F64, REF32, C32, C32K, F64F = 0, 1, 2, 3, 4
 BC_REFLECT, BC_OUTFLOW = 0, 1
 LIM_MINMOD, LIM_MC = 0, 1
 

@@ -25,6 +25,12 @@
  *       equals the 1D run, L1 convergence to the exact Riemann solution.
  *   C32: no reference exists (it is this project's compensated scheme); pinned by
  *       the same invariants and by its distance to F64 (L1(rho) <= 1e-6).
+ *   F64F: no reference exists (the C32 arrangement in double); pinned by the same
+ *       invariants (rest state, F+ + F- = F, M = 0 closed form, mirror symmetry,
+ *       conservation, wall impulse, exact Riemann convergence) and by its distance to
+ *       the F64 mode, which is FP64 round-off only: elementwise relative 1e-13 after
+ *       short runs of either order and L1(rho) <= 1e-10 at t = 0.2 on second-order
+ *       tubes, where a wrong sign, index or dropped term shows as 1e-4 or more.
  *   Initial conditions: bitwise against the reference's own initial arrays
  *       (golden step-0 dumps) for Sod, implosion and four shocks; the circle by a
  *       brute-force count of cells inside the radius.
@@ -402,6 +408,157 @@ static int c32_run(const orc_problem *pb, int kahan, const double *prim_in, int6
 }
 
 /* ------------------------------------------------------------------------- */
+/* FP64 face form (F64F, 1D).  The compensated mode's arrangement with every */
+/* value a double: 1/rho once and products by it, fused multiply-adds where   */
+/* a product feeds a sum (fma is correctly rounded in C99/glibc), correctly   */
+/* rounded sqrt and u/a, minmod as the median of (l, r, 0), face terms        */
+/* t = lam*G (second order: fma(lam/2, d, lam*G)) added to the state as       */
+/* u += t_left - t_right.  The initial state is the F64 mode's.               */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+    int nx, n, order, bc, lim, ke;
+    double gR, rcv, cv, lamx, hlamx, theta;
+    double *u[3], *fp[3], *fm[3], *a;
+    void *block;
+} f64fgrid;
+
+static double mm_f64f(double l, double r)
+{
+    if (l > 0.0 && r > 0.0) return l < r ? l : r;
+    if (l < 0.0 && r < 0.0) return l > r ? l : r;
+    return 0.0;
+}
+
+static int f64f_alloc(f64fgrid *g, const orc_problem *pb, const double *prim)
+{
+    cst_f64 k;
+    int v, c;
+    size_t n = (size_t)pb->nx;
+    double *base;
+    memset(g, 0, sizeof(*g));
+    g->nx = pb->nx; g->n = (int)n;
+    g->order = pb->order; g->bc = pb->bc; g->lim = pb->limiter;
+    g->ke = pb->order == 1 ? 0 : 1;
+    g->theta = pb->theta;
+    make_cst_f64(&k, pb->nx, 1, pb->lx, 1.0);
+    g->gR = k.gR;
+    g->cv = k.cv;
+    g->rcv = 1.0 / k.cv;
+    g->lamx = k.dtdx;
+    g->hlamx = 0.5 * k.dtdx;
+    g->block = malloc(n * sizeof(double) * 10 + 64);
+    if (!g->block) return -1;
+    base = (double *)g->block;
+    for (v = 0; v < 3; v++) {
+        g->u[v] = base; base += n;
+        g->fp[v] = base; base += n;
+        g->fm[v] = base; base += n;
+    }
+    g->a = base;
+    for (c = 0; c < g->n; c++) {
+        double p[3], u[3];
+        for (v = 0; v < 3; v++) p[v] = prim[(size_t)v * g->n + c];
+        p2u_f64(1, g->ke, g->cv, p, u);
+        for (v = 0; v < 3; v++) g->u[v][c] = u[v];
+    }
+    return 0;
+}
+
+/* split fluxes of one cell (base_shll.c:135-157 in the F64F arrangement) */
+static void f64f_cell_flux(f64fgrid *g, int c)
+{
+    int v;
+    double r = g->u[0][c], m = g->u[1][c], E = g->u[2][c];
+    double inv = 1.0 / r;
+    double vx = m * inv, e = E * inv;
+    double T = fma(-0.5 * vx, vx, e) * g->rcv;
+    double a = sqrt(g->gR * T);
+    double M = vx / a;
+    double z1 = fma(0.5, M, 0.5), z2 = (0.5 * a) * fma(-M, M, 1.0), z3 = fma(0.5, M, -0.5);
+    double P = r * T;
+    double f[3], U[3];
+    g->a[c] = a;
+    f[0] = m; f[1] = fma(m, vx, P); f[2] = vx * (E + P);
+    U[0] = r; U[1] = m; U[2] = E;
+    for (v = 0; v < 3; v++) {
+        double uz = U[v] * z2;
+        g->fp[v][c] = fma(f[v], z1, uz);
+        g->fm[v][c] = fma(-f[v], z3, -uz);
+    }
+}
+
+static double f64f_slope(const f64fgrid *g, const double *q, int s)
+{
+    double m;
+    if (s <= 0 || s >= g->nx - 1) return 0.0;
+    m = mm_f64f(q[s] - q[s - 1], q[s + 1] - q[s]);
+    if (g->lim != ORC_LIM_MC) return m;
+    return mm_f64f(0.5 * (q[s + 1] - q[s - 1]), g->theta * m);
+}
+
+/* face term of face k (between cells k-1 and k), as c32_face in double */
+static double f64f_face(const f64fgrid *g, int v, int k)
+{
+    static const double sgn[3] = {-1.0, 1.0, -1.0};
+    const double *p = g->fp[v], *m = g->fm[v];
+    double G, d = 0.0;
+    int len = g->nx;
+    if (k == 0) {
+        G = g->bc == ORC_BC_REFLECT ? m[0] + sgn[v] * m[0] : p[0] + m[0];
+    } else if (k == len) {
+        G = g->bc == ORC_BC_REFLECT ? p[len - 1] + sgn[v] * p[len - 1] : p[len - 1] + m[len - 1];
+    } else {
+        G = p[k - 1] + m[k];
+        if (g->order == 2) d = f64f_slope(g, p, k - 1) - f64f_slope(g, m, k);
+    }
+    return g->order == 2 ? fma(g->hlamx, d, g->lamx * G) : g->lamx * G;
+}
+
+static void f64f_step(f64fgrid *g)
+{
+    int c, v;
+    for (c = 0; c < g->n; c++) f64f_cell_flux(g, c);
+    for (v = 0; v < 3; v++) {
+        double tl = f64f_face(g, v, 0);
+        for (c = 0; c < g->n; c++) {
+            double tr = f64f_face(g, v, c + 1);
+            g->u[v][c] = g->u[v][c] + (tl - tr);
+            tl = tr;
+        }
+    }
+}
+
+static void f64f_output(const f64fgrid *g, double *prim_out, double *cons_out)
+{
+    int c, v;
+    for (c = 0; c < g->n; c++) {
+        double u[3], p[3];
+        for (v = 0; v < 3; v++) u[v] = g->u[v][c];
+        u2p_f64(1, g->ke, g->cv, u, p);
+        for (v = 0; v < 3; v++) {
+            if (cons_out) cons_out[(size_t)v * g->n + c] = u[v];
+            if (prim_out) prim_out[(size_t)v * g->n + c] = p[v];
+        }
+    }
+}
+
+static int f64f_run(const orc_problem *pb, const double *prim_in, int64_t nsteps, double *prim_out,
+                    double *cons_out, double *elapsed)
+{
+    f64fgrid g;
+    int64_t s;
+    double t0;
+    if (pb->dim != 1) return -1;
+    if (f64f_alloc(&g, pb, prim_in)) return -2;
+    t0 = orc_now();
+    for (s = 0; s < nsteps; s++) f64f_step(&g);
+    if (elapsed) *elapsed = orc_now() - t0;
+    f64f_output(&g, prim_out, cons_out);
+    free(g.block);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------- */
 int orc_run(const orc_problem *pb, int mode, const double *prim_in, int64_t nsteps,
             double *prim_out, double *cons_out, int nthreads, double *elapsed_s)
 {
@@ -411,6 +568,7 @@ int orc_run(const orc_problem *pb, int mode, const double *prim_in, int64_t nste
     case ORC_F64: return run_f64(pb, prim_in, nsteps, prim_out, cons_out, nthreads, elapsed_s);
     case ORC_C32: return c32_run(pb, 0, prim_in, nsteps, prim_out, cons_out, NULL, NULL, elapsed_s);
     case ORC_C32K: return c32_run(pb, 1, prim_in, nsteps, prim_out, cons_out, NULL, NULL, elapsed_s);
+    case ORC_F64F: return f64f_run(pb, prim_in, nsteps, prim_out, cons_out, elapsed_s);
     default: return -1;
     }
 }
@@ -488,6 +646,22 @@ int orc_stage(const orc_problem *pb, int mode, const double *prim_in, double *co
             for (c = 0; c < g.n; c++) a[c] = g.a[c];
         if (prim) c32_output(&g, prim, NULL);
         free(g.block);
+    } else if (mode == ORC_F64F && pb->dim == 1) {
+        f64fgrid g;
+        int v, c;
+        if (f64f_alloc(&g, pb, prim_in)) return -2;
+        for (c = 0; c < g.n; c++) f64f_cell_flux(&g, c);
+        for (v = 0; v < 3; v++)
+            for (c = 0; c < g.n; c++) {
+                size_t o = (size_t)v * g.n + c;
+                if (cons) cons[o] = g.u[v][c];
+                if (fp) fp[o] = g.fp[v][c];
+                if (fm) fm[o] = g.fm[v][c];
+            }
+        if (a)
+            for (c = 0; c < g.n; c++) a[c] = g.a[c];
+        if (prim) f64f_output(&g, prim, NULL);
+        free(g.block);
     } else {
         return -1;
     }
@@ -499,6 +673,7 @@ double orc_minmod(int mode, double l, double r)
 {
     if (mode == ORC_REF32) return (double)mm_r32((float)l, (float)r);
     if (mode == ORC_C32 || mode == ORC_C32K) return (double)mm_c32((float)l, (float)r);
+    if (mode == ORC_F64F) return mm_f64f(l, r);
     return mm_f64(l, r);
 }
 
@@ -514,6 +689,12 @@ double orc_slope(int mode, int limiter, double theta, double ql, double qc, doub
         float q[3] = {(float)ql, (float)qc, (float)qr};
         g.lim = limiter; g.theta = (float)theta;
         return (double)c32_slope(&g, q, 1, 1, 1, 3);
+    }
+    if (mode == ORC_F64F) {
+        f64fgrid g;
+        double q[3] = {ql, qc, qr};
+        g.nx = 3; g.lim = limiter; g.theta = theta;
+        return f64f_slope(&g, q, 1);
     }
     {
         grid_f64 g;

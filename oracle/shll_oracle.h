@@ -18,7 +18,7 @@
  * be timed as the CPU baseline, serially or with one OpenMP worksharing loop
  * per pass (the base-omp pattern).
  *
- * Three arithmetic modes:
+ * Arithmetic modes:
  *   ORC_REF32 : float storage, with the double promotions the reference's
  *               double literals and double sqrt() cause.  Pinned bitwise to
  *               the compiled reference (tests/golden).
@@ -30,6 +30,9 @@
  *               judged against ORC_F64.
  *   ORC_C32K  : ORC_C32 with the face terms added to each cell by one Kahan
  *               (compensated summation) step instead of the TwoSum chain.
+ *   ORC_F64F  : 1D only: ORC_C32K's formulas with every value a double and the
+ *               face terms added to the FP64 state directly (no compensation).
+ *               Restates the product's fast FP64 kernel; judged against ORC_F64.
  *
  * Parity pinning status (which check fixes which function) is listed per
  * function in shll_oracle.c.
@@ -43,7 +46,7 @@
 extern "C" {
 #endif
 
-enum { ORC_F64 = 0, ORC_REF32 = 1, ORC_C32 = 2, ORC_C32K = 3 };
+enum { ORC_F64 = 0, ORC_REF32 = 1, ORC_C32 = 2, ORC_C32K = 3, ORC_F64F = 4 };
 enum { ORC_BC_REFLECT = 0, ORC_BC_OUTFLOW = 1 };
 /* Slope limiters of the second-order scheme:
  *   ORC_LIM_MINMOD : minmod of the one-sided differences (base-c/2nd_order_base_shll.c:268-276)

@@ -34,7 +34,7 @@ hipError_t launch_output(int prec, int dim, int nblocks, int n_inst, hipStream_t
                          const void* s0, const void* s1, int64_t plane, double* prim, double* cons);
 hipError_t launch_sums(int prec, int dim, int nblocks, int n_inst, hipStream_t st, const Inst* insts,
                        const void* s0, const void* s1, int64_t plane, double* partial);
-hipError_t launch_ctl(hipStream_t st, int64_t* ctl, int64_t value, int add);
+hipError_t launch_ctl(hipStream_t st, int64_t* ctl, int64_t v0, int64_t v1, int64_t v2, int add);
 hipError_t launch_compare(int nblocks, hipStream_t st, const double* a, int64_t plane_a, const double* b,
                           int64_t plane_b, int64_t ncell, int dim, double* partial);
 }  // namespace shll
@@ -96,6 +96,7 @@ static Consts make_consts(int64_t nx, int64_t ny, double lx, double ly, double t
         k.d_R = R;
         k.d_lamx = CFL / (R + 1.0);
         k.d_lamy = DT / DY;
+        k.d_rcv = 1.0 / k.d_cv;
     }
     k.c_rcv = (float)(1.0 / k.d_cv);
     k.c_gR = (float)k.d_gR;
@@ -117,9 +118,12 @@ extern "C" int64_t shll_steps_to_time(const shll_problem* pb, float t_end) {
     float time = 0.0f;
     int64_t n = 0;
     while (time < t_end) {  // base_shll.c:208-218
-        time += DT;
+        const float next = time + DT;
+        // once DT is below half an ulp of time the float sum stops growing and the
+        // reference's loop would never end: report that instead of spinning
+        if (!(next > time)) return -1;
+        time = next;
         n++;
-        if (n > ((int64_t)1 << 40)) return -1;
     }
     return n;
 }
@@ -129,8 +133,9 @@ static shll_status check_problem(const shll_problem* pb) {
     if (pb->dim != 1 && pb->dim != 2) return fail(SHLL_EINVAL, "dim must be 1 or 2 (got %d)", pb->dim);
     if (pb->order != 1 && pb->order != 2) return fail(SHLL_EINVAL, "order must be 1 or 2 (got %d)", pb->order);
     if (pb->bc != SHLL_BC_REFLECT && pb->bc != SHLL_BC_OUTFLOW) return fail(SHLL_EINVAL, "bad bc %d", pb->bc);
-    if (pb->precision < SHLL_F64 || pb->precision > SHLL_C32K)
+    if (pb->precision < SHLL_F64 || pb->precision > SHLL_F64F)
         return fail(SHLL_EINVAL, "bad precision %d", pb->precision);
+    if (pb->precision == SHLL_F64F && pb->dim != 1) return fail(SHLL_EINVAL, "SHLL_F64F is a 1D form");
     if (pb->nx < 2 || pb->nx > (1 << 30)) return fail(SHLL_EINVAL, "nx out of range (%lld)", (long long)pb->nx);
     if (pb->dim == 1 && pb->ny != 1) return fail(SHLL_EINVAL, "ny must be 1 in 1D");
     if (pb->dim == 2 && (pb->ny < 2 || pb->ny > (1 << 30))) return fail(SHLL_EINVAL, "ny out of range");
@@ -153,7 +158,7 @@ static shll_status ensure_device() {
     return SHLL_OK;
 }
 
-static size_t word_bytes(int prec) { return prec == SHLL_F64 ? 8 : 4; }
+static size_t word_bytes(int prec) { return prec == SHLL_F64 || prec == SHLL_F64F ? 8 : 4; }
 static int nwords(int prec) { return prec == SHLL_C32 || prec == SHLL_C32K ? 2 : 1; }
 
 // ---------------------------------------------------------------------------
@@ -251,12 +256,13 @@ struct Engine {
     void* d_items = nullptr;
     size_t d_items_cap = 0;
     int64_t n_items = 0;
-    std::vector<int64_t> items_active;  // items_active[j] = #items whose instance has nsteps > j*k (by distinct
-                                        // instance step counts); computed on demand
     std::vector<int64_t> inst_first_item;  // prefix: items of instance rank r start at inst_first_item[r]
     bool items_valid = false;
     // progress
-    int64_t launches_done = 0;  // launches since the initial state (batch: common step_base = launches*k)
+    int64_t launches_done = 0;  // launches since the initial state (their parity names the buffer of
+                                // the instances still active)
+    int64_t steps_done = 0;     // batch: steps every unfinished instance has taken
+    std::vector<int64_t> launch_starts;  // batch: first step of every launch so far (ascending)
     int cur = 0;                // buffer holding the state of instances active now
     // graph (single-instance constant-parameter graphs, by buffer parity)
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
@@ -272,11 +278,11 @@ struct Engine {
         int64_t blocks = 0;
     };
     std::vector<BatchGraph> bgs;
-    int64_t* d_ctl = nullptr;
-    int32_t* d_active = nullptr;
+    int64_t* d_ctl = nullptr;           // {j0, anchor, jmax}, see StepArgs
+    int32_t* d_active = nullptr;        // item counts of the anchored launch grid
     size_t active_cap = 0;
-    int64_t jmax = 0;  // launches of a whole batch run
-    int b_key[3] = {0, 0, 0};
+    int64_t anchor = -1;                // first step of the grid d_active describes (-1: none)
+    std::vector<int32_t> h_active;      // host copy of d_active (source of its upload)
     // diagnostics scratch
     double* d_prim = nullptr;
     double* d_cons = nullptr;
@@ -329,7 +335,10 @@ static int radius(const Engine* E) { return E->order; }
 // steps this launch shape advances per launch
 static int launch_k(const Engine* E) { return E->L.k; }
 
-static int64_t active_items(const Engine* E, int64_t j);
+static int64_t active_items(const Engine* E, int64_t step_base);
+static int64_t batch_total_steps(const Engine* E) {
+    return E->order_idx.empty() ? 0 : E->h_inst[E->order_idx[0]].nsteps;
+}
 
 static shll_status engine_build_items(Engine* E) {
     if (E->items_valid) return SHLL_OK;
@@ -362,33 +371,32 @@ static shll_status engine_build_items(Engine* E) {
     }
     HIPCHECK(hipMemcpyAsync(E->d_items, E->dim == 1 ? (const void*)it1.data() : (const void*)it2.data(), bytes,
                             hipMemcpyHostToDevice, E->stream));
+    HIPCHECK(hipStreamSynchronize(E->stream));
+    // the graphs hold the old pointers: drop them before any buffer they read is freed
+    engine_drop_graphs(E);
     if (E->batch) {
-        // active-item count per launch index, read by the constant graph nodes
-        const int64_t total = E->order_idx.empty() ? 0 : E->h_inst[E->order_idx[0]].nsteps;
-        E->jmax = (total + launch_k(E) - 1) / launch_k(E);
-        if (E->jmax > (int64_t)std::numeric_limits<int32_t>::max())
+        // item counts of a grid of k-step launches (at most one launch per k steps of the
+        // longest instance, whatever step it starts at), read by the constant graph nodes
+        const int64_t jcap = batch_total_steps(E) / launch_k(E) + 2;
+        if (jcap > (int64_t)std::numeric_limits<int32_t>::max())
             return fail(SHLL_EINVAL, "too many launches for one batch run");
-        std::vector<int32_t> act((size_t)std::max<int64_t>(1, E->jmax), 0);
-        for (int64_t j = 0; j < E->jmax; ++j) act[(size_t)j] = (int32_t)active_items(E, j);
-        const size_t abytes = act.size() * sizeof(int32_t);
+        const size_t abytes = (size_t)jcap * sizeof(int32_t);
         if (abytes > E->active_cap) {
             if (E->d_active) HIPCHECK(hipFree(E->d_active));
             E->d_active = nullptr;
             HIPCHECK(hipMalloc(&E->d_active, abytes));
             E->active_cap = abytes;
         }
-        if (!E->d_ctl) HIPCHECK(hipMalloc(&E->d_ctl, sizeof(int64_t)));
-        HIPCHECK(hipMemcpyAsync(E->d_active, act.data(), abytes, hipMemcpyHostToDevice, E->stream));
+        if (!E->d_ctl) HIPCHECK(hipMalloc(&E->d_ctl, 3 * sizeof(int64_t)));
+        E->anchor = -1;
     }
-    HIPCHECK(hipStreamSynchronize(E->stream));
     E->items_valid = true;
-    engine_drop_graphs(E);
     return SHLL_OK;
 }
 
-// number of items whose instance is still active at launch j (instances sorted by nsteps)
-static int64_t active_items(const Engine* E, int64_t j) {
-    const int64_t base = j * (int64_t)launch_k(E);
+// number of items whose instance still has steps to take at step step_base (instances
+// sorted by nsteps)
+static int64_t active_items(const Engine* E, int64_t base) {
     // instances in order_idx are sorted by nsteps descending: find first with nsteps <= base
     size_t lo = 0, hi = E->order_idx.size();
     while (lo < hi) {
@@ -399,13 +407,13 @@ static int64_t active_items(const Engine* E, int64_t j) {
     return E->inst_first_item[lo];
 }
 
-static StepArgs make_args(const Engine* E, int64_t j, int parity, int64_t n_items) {
+static StepArgs make_args(const Engine* E, int64_t step_base, int parity, int64_t n_items) {
     StepArgs a;
     a.inst = E->d_inst;
     a.items = E->d_items;
     a.n_items = (int32_t)n_items;
     a.ksteps = launch_k(E);
-    a.step_base = j * (int64_t)launch_k(E);
+    a.step_base = step_base;
     a.plane = E->plane;
     a.in0 = E->buf[parity][0];
     a.in1 = E->buf[parity][1];
@@ -414,7 +422,6 @@ static StepArgs make_args(const Engine* E, int64_t j, int parity, int64_t n_item
     a.jctl = nullptr;
     a.active = nullptr;
     a.jofs = 0;
-    a.jmax = 0;
     return a;
 }
 
@@ -485,9 +492,11 @@ static shll_status single_steps(Engine* E, int64_t nsteps) {
     return SHLL_OK;
 }
 
-// Batch: launch j covers steps [j*k, (j+1)*k) of every instance with nsteps > j*k.
-// Whole chunks of g launches replay constant graphs (device-driven launch index); the
-// remainder of a call is launched directly with host-side parameters.
+// Batch: a launch at step s covers steps [s, s + r) (r <= k) of every instance with
+// nsteps > s.  Whole chunks of g k-step launches replay constant graphs whose launches
+// take their step base and item count from the device counter and the item-count table of
+// a grid of k-step launches anchored at some step; the rest of a call is launched directly
+// with host-side parameters, the last launch with the steps that remain.
 static shll_status batch_graph(Engine* E, int parity, int64_t blocks, int g, hipGraphExec_t* out) {
     for (const auto& bgx : E->bgs)
         if (bgx.parity == parity && bgx.blocks == blocks) {
@@ -502,10 +511,9 @@ static shll_status batch_graph(Engine* E, int parity, int64_t blocks, int g, hip
         a.jctl = E->d_ctl;
         a.active = E->d_active;
         a.jofs = q;
-        a.jmax = (int32_t)E->jmax;
         e = launch_one(E, a, blocks * items_per_block(E));
     }
-    if (e == hipSuccess) e = launch_ctl(E->stream, E->d_ctl, g, 1);
+    if (e == hipSuccess) e = launch_ctl(E->stream, E->d_ctl, g, 0, 0, 1);
     hipError_t e2 = hipStreamEndCapture(E->stream, &graph);
     if (e != hipSuccess || e2 != hipSuccess)
         return fail(SHLL_EDEVICE, "batch graph capture: %s", hipGetErrorString(e != hipSuccess ? e : e2));
@@ -520,44 +528,69 @@ static shll_status batch_graph(Engine* E, int parity, int64_t blocks, int g, hip
     return SHLL_OK;
 }
 
-static shll_status batch_launches(Engine* E, int64_t nlaunch) {
+// anchor the graph launches' grid at step s: the item count of every k-step launch from s
+// to the end of the longest instance, uploaded to d_active (stream-ordered after every
+// launch that read the previous table)
+static shll_status batch_set_anchor(Engine* E, int64_t s) {
     const int k = launch_k(E);
-    int64_t j = E->launches_done;
-    const int64_t jend = std::min<int64_t>(E->jmax, j + nlaunch);
+    const int64_t jmax = (batch_total_steps(E) - s + k - 1) / k;
+    E->h_active.assign((size_t)std::max<int64_t>(1, jmax), 0);
+    if ((size_t)E->h_active.size() * sizeof(int32_t) > E->active_cap)
+        return fail(SHLL_EINVAL, "internal: launch grid exceeds its table");
+    for (int64_t j = 0; j < jmax; ++j) E->h_active[(size_t)j] = (int32_t)active_items(E, s + j * k);
+    HIPCHECK(hipMemcpyAsync(E->d_active, E->h_active.data(), E->h_active.size() * sizeof(int32_t),
+                            hipMemcpyHostToDevice, E->stream));
+    E->anchor = s;
+    return SHLL_OK;
+}
+
+static shll_status batch_launches(Engine* E, int64_t nsteps) {
+    const int k = launch_k(E);
+    int64_t s = E->steps_done;
+    const int64_t send = std::min<int64_t>(batch_total_steps(E), s + nsteps);
     const int g = E->L.graph > 0 ? std::max(2, (E->L.graph / k) & ~1) : 0;
     const int ipb = items_per_block(E);
     int64_t launches = 0;
     bool ctl_set = false;
     if (E->timing) HIPCHECK(hipEventRecord(E->ev0, E->stream));
-    while (j < jend) {
-        if (g > 0 && jend - j >= g) {
+    while (s < send) {
+        if (g > 0 && send - s >= (int64_t)g * k) {
             if (!ctl_set) {
-                HIPCHECK(launch_ctl(E->stream, E->d_ctl, j, 0));
+                if (E->anchor < 0 || E->anchor > s || (s - E->anchor) % k != 0) {
+                    shll_status st = batch_set_anchor(E, s);
+                    if (st != SHLL_OK) return st;
+                }
+                const int64_t jmax = (batch_total_steps(E) - E->anchor + k - 1) / k;
+                HIPCHECK(launch_ctl(E->stream, E->d_ctl, (s - E->anchor) / k, E->anchor, jmax, 0));
                 ctl_set = true;
             }
             // the chunk's first launch has the most active items; round its grid up to a
             // power of two so few graphs cover the whole run
-            const int64_t need = (active_items(E, j) + ipb - 1) / ipb;
+            const int64_t need = (active_items(E, s) + ipb - 1) / ipb;
             int64_t blocks = 1;
             while (blocks < need) blocks <<= 1;
             hipGraphExec_t exec = nullptr;
-            shll_status st = batch_graph(E, (int)(j & 1), blocks, g, &exec);
+            shll_status st = batch_graph(E, (int)((E->launches_done + launches) & 1), blocks, g, &exec);
             if (st != SHLL_OK) return st;
             HIPCHECK(hipGraphLaunch(exec, E->stream));
-            j += g;
+            for (int q = 0; q < g; ++q) E->launch_starts.push_back(s + (int64_t)q * k);
+            s += (int64_t)g * k;
             launches += g;
             continue;
         }
-        const int64_t na = active_items(E, j);
-        StepArgs a = make_args(E, j, (int)(j & 1), na);
+        const int64_t na = active_items(E, s);
+        StepArgs a = make_args(E, s, (int)((E->launches_done + launches) & 1), na);
+        a.ksteps = (int32_t)std::min<int64_t>(k, send - s);
         HIPCHECK(launch_one(E, a, na));
-        j++;
+        E->launch_starts.push_back(s);
+        s += a.ksteps;
         launches++;
     }
     if (E->timing) HIPCHECK(hipEventRecord(E->ev1, E->stream));
     HIPCHECK(hipStreamSynchronize(E->stream));
-    E->launches_done = j;
-    E->cur = (int)(j & 1);
+    E->steps_done = s;
+    E->launches_done += launches;
+    E->cur = (int)(E->launches_done & 1);
     E->last_launches = launches;
     if (E->timing && launches > 0) {
         float ms = 0;
@@ -568,13 +601,21 @@ static shll_status batch_launches(Engine* E, int64_t nlaunch) {
     return SHLL_OK;
 }
 
+// back to the initial state's bookkeeping (the caller restores the state itself)
+static void engine_rewind(Engine* E) {
+    E->cur = 0;
+    E->launches_done = 0;
+    E->steps_done = 0;
+    E->launch_starts.clear();
+}
+
 // buffer holding instance i's current state (batch bookkeeping)
 static int inst_parity(const Engine* E, int64_t i) {
     if (!E->batch) return E->cur;
-    const int k = launch_k(E);
-    const int64_t J = (E->h_inst[i].nsteps + k - 1) / k;  // launches instance i takes in total
-    const int64_t done = std::min<int64_t>(J, E->launches_done);
-    return (int)(done & 1);
+    // the launches instance i took part in: those that started before its last step
+    const auto& st = E->launch_starts;
+    const int64_t took = std::lower_bound(st.begin(), st.end(), E->h_inst[i].nsteps) - st.begin();
+    return (int)(took & 1);
 }
 
 static shll_status engine_alloc(Engine* E, int64_t plane) {
@@ -628,8 +669,7 @@ static shll_status engine_init_state(Engine* E, const double* const* prims) {
     (void)hipFree(d_p);
     if (e != hipSuccess || e2 != hipSuccess)
         return fail(SHLL_EDEVICE, "init kernel: %s", hipGetErrorString(e != hipSuccess ? e : e2));
-    E->cur = 0;
-    E->launches_done = 0;
+    engine_rewind(E);
     return SHLL_OK;
 }
 
@@ -805,7 +845,9 @@ extern "C" shll_status shll_step(shll_ctx* ctx, int64_t nsteps) {
 extern "C" shll_status shll_run_to_time(shll_ctx* ctx, float t_end, int64_t* steps_out) {
     if (!ctx) return fail(SHLL_EINVAL, "ctx is NULL");
     const int64_t n = shll_steps_to_time(&ctx->pb, t_end);
-    if (n < 0) return fail(SHLL_EINVAL, "bad t_end");
+    if (n < 0)
+        return fail(SHLL_EINVAL, "the float time loop never reaches t_end = %g: DT is below half an ulp of the "
+                                 "time before it gets there", (double)t_end);
     shll_status st = shll_step(ctx, n);
     if (st != SHLL_OK) return st;
     if (steps_out) *steps_out = n;
@@ -1187,17 +1229,14 @@ static shll_status engine_autotune(Engine* E, int32_t trial_steps, int64_t shape
     const bool was_timing = E->timing;
     E->timing = false;
     for (const Launch& L : cands) {
-        E->cur = 0;
-        E->launches_done = 0;
+        engine_rewind(E);
         if (set_launch(E, L.C, L.k, L.wpb, L.graph) != SHLL_OK) continue;
         if ((st = engine_snapshot(E, false)) != SHLL_OK) break;
-        E->cur = 0;
-        E->launches_done = 0;
+        engine_rewind(E);
         // warm-up (also captures graphs), then the timed trial
         if ((st = advance(E, std::max<int64_t>(1, trial_steps / 4))) != SHLL_OK) break;
         if ((st = engine_snapshot(E, false)) != SHLL_OK) break;
-        E->cur = 0;
-        E->launches_done = 0;
+        engine_rewind(E);
         auto t0 = std::chrono::steady_clock::now();
         if ((st = advance(E, trial_steps)) != SHLL_OK) break;
         const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -1207,12 +1246,10 @@ static shll_status engine_autotune(Engine* E, int32_t trial_steps, int64_t shape
         }
     }
     E->timing = was_timing;
-    E->cur = 0;
-    E->launches_done = 0;
+    engine_rewind(E);
     shll_status st2 = set_launch(E, bestL.C, bestL.k, bestL.wpb, bestL.graph);
     shll_status st3 = engine_snapshot(E, false);
-    E->cur = 0;
-    E->launches_done = 0;
+    engine_rewind(E);
     if (st != SHLL_OK) return st;
     if (st2 != SHLL_OK) return st2;
     if (st3 != SHLL_OK) return st3;
@@ -1305,10 +1342,7 @@ extern "C" void shll_batch_free(shll_batch* b) {
     delete b;
 }
 
-static shll_status batch_advance_steps(Engine* E, int64_t nsteps) {
-    const int k = launch_k(E);
-    return batch_launches(E, (nsteps + k - 1) / k);
-}
+static shll_status batch_advance_steps(Engine* E, int64_t nsteps) { return batch_launches(E, nsteps); }
 
 extern "C" shll_status shll_batch_advance(shll_batch* b, int64_t nsteps, int64_t* max_remaining) {
     if (!b || nsteps < 0) return fail(SHLL_EINVAL, "bad argument");
@@ -1319,8 +1353,7 @@ extern "C" shll_status shll_batch_advance(shll_batch* b, int64_t nsteps, int64_t
     st = batch_advance_steps(E, nsteps);
     if (st != SHLL_OK) return st;
     if (max_remaining) {
-        const int64_t total = E->h_inst[E->order_idx[0]].nsteps;
-        *max_remaining = std::max<int64_t>(0, total - E->launches_done * launch_k(E));
+        *max_remaining = std::max<int64_t>(0, batch_total_steps(E) - E->steps_done);
     }
     return SHLL_OK;
 }
@@ -1406,8 +1439,7 @@ extern "C" shll_status shll_batch_reset(shll_batch* b) {
     if (!b) return fail(SHLL_EINVAL, "batch is NULL");
     HIPCHECK(hipSetDevice(b->E->device));
     shll_status st = engine_snapshot(b->E, false);
-    b->E->cur = 0;
-    b->E->launches_done = 0;
+    engine_rewind(b->E);
     return st;
 }
 

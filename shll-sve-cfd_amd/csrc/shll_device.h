@@ -7,6 +7,8 @@
 //   PREC_C32   : float arithmetic, (hi, lo) float state, conservative face update
 //                with TwoSum (the FP32-compensated form)
 //   PREC_C32K  : PREC_C32 with one Kahan step per cell for the update
+//   PREC_F64F  : FP64 values in the compensated forms' arrangement (fused multiply-adds,
+//                1/rho once, face-term update); 1D
 //
 // The library is compiled with -ffp-contract=off: every + - * below is one IEEE-754
 // operation in source order (fused multiply-adds only where written), and division
@@ -244,6 +246,42 @@ struct C32Arith {
     __device__ static __forceinline__ float mm(float l, float r) { return __builtin_amdgcn_fmed3f(l, r, 0.0f); }
     __device__ static __forceinline__ float half(float x) { return 0.5f * x; }
 };
+
+// FP64 in the compensated scheme's arrangement (PREC_F64F): the C32 formulas with every
+// value a double -- fused multiply-adds where a product feeds a sum, 1/rho once (correctly
+// rounded) and products by it, correctly rounded sqrt and u/a.  Not the reference's
+// association, so not bitwise the FP64 oracle; its distance to it is FP64 round-off grown
+// by the scheme (about 1e-12 in L1 on the second-order tubes).
+struct F64FArith {
+    __device__ static __forceinline__ void prim1(double u0, double u1, double u2, double rcv, double gR,
+                                                 double& vx, double& T, double& a) {
+        const double inv = div_fast(1.0, u0);
+        vx = u1 * inv;
+        const double e = u2 * inv;
+        T = __builtin_fma(-0.5 * vx, vx, e) * rcv;  // (e - vx^2/2) / cv
+        a = sqrt_fast(gR * T);
+    }
+    // z1, z3 = RN(M/2 +- 1/2) (M/2 is exact), z2 = (a/2)(1 - M^2) with 1 - M^2 fused
+    __device__ static __forceinline__ void zinv(double M, double a, double& z1, double& z2, double& z3) {
+        z1 = __builtin_fma(0.5, M, 0.5);
+        z2 = (0.5 * a) * __builtin_fma(-M, M, 1.0);
+        z3 = __builtin_fma(0.5, M, -0.5);
+    }
+    // minmod as the median of (l, r, 0): the smaller magnitude when the signs agree, zero
+    // otherwise (sign of a zero result unspecified, as in C32Arith::mm)
+    __device__ static __forceinline__ double mm(double l, double r) {
+        return __builtin_fmax(__builtin_fmin(l, r), __builtin_fmin(__builtin_fmax(l, r), 0.0));
+    }
+    __device__ static __forceinline__ double half(double x) { return 0.5 * x; }
+};
+
+// the same split in FP64: U*Z2 once, two fused multiply-adds
+__device__ __forceinline__ void split_f64f(double f, double u, double z1, double z2, double z3, double& fp,
+                                           double& fm) {
+    const double uz = u * z2;
+    fp = __builtin_fma(f, z1, uz);
+    fm = __builtin_fma(-f, z3, -uz);
+}
 
 // Limited slope of a split flux from its values at c-1, c, c+1 (interior cells):
 //   LIM_MINMOD  mm(qc - ql, qr - qc)                                 2nd_order_base_shll.c:268-276

@@ -19,7 +19,15 @@ template <> struct Real<PREC_C32K> { using T = float; };
 template <int PREC, class RT>
 __device__ __forceinline__ void flux1(RT u0, RT u1, RT u2, const Inst& in, RT (&fp)[3], RT (&fm)[3], RT& a) {
     RT vx, T, z1, z2, z3;
-    if constexpr (PREC == PREC_C32 || PREC == PREC_C32K) {
+    if constexpr (PREC == PREC_F64F) {
+        F64FArith::prim1(u0, u1, u2, in.k.d_rcv, in.k.d_gR, vx, T, a);
+        F64FArith::zinv(div_fast(vx, a), a, z1, z2, z3);
+        const RT P = u0 * T;  // R = 1
+        const RT f0 = u1, f1 = __builtin_fma(u1, vx, P), f2 = vx * (u2 + P);
+        split_f64f(f0, u0, z1, z2, z3, fp[0], fm[0]);
+        split_f64f(f1, u1, z1, z2, z3, fp[1], fm[1]);
+        split_f64f(f2, u2, z1, z2, z3, fp[2], fm[2]);
+    } else if constexpr (PREC == PREC_C32 || PREC == PREC_C32K) {
         C32Arith::prim1(u0, u1, u2, in.k.c_rcv, in.k.c_gR, vx, T, a);
         C32Arith::zinv(div_fast(vx, a), a, z1, z2, z3);
         const RT P = u0 * T;  // R = 1

@@ -14,6 +14,9 @@
 // instead hold consecutive segments of one item and swap their edge fluxes through LDS
 // once a step (one barrier), so only the block's ends carry halo.
 //
+// Precisions: F64 / REF32 in the reference's association (cell by cell), C32 / C32K and
+// F64F in face-term form (one term per face, shared by both cells).
+//
 // References: base_shll.c:87-177 (first order, reflective), 2nd_order_base_shll.c
 // :203-459 (second order, outflow; its x sweep on the 3-variable system is the 1D
 // second-order scheme), base-omp/2nd_order_base_shll.c:317-325 (the MC limiter).
@@ -318,6 +321,58 @@ __device__ __forceinline__ void step_c32(float (&hi)[3][C], float (&lo)[3][C], c
 }
 
 // ---------------------------------------------------------------------------
+// F64F: the C32 face-term arrangement with every value a double (no compensation: the
+// face terms are added to the FP64 state directly, u += t[m] - t[m+1])
+// ---------------------------------------------------------------------------
+template <int ORDER, int C, int WM, int LIM, int XW>
+__device__ __forceinline__ void step_f64f(double (&u)[3][C], const Inst& in, int gl, const Coop<double, XW>& x,
+                                          int parity) {
+    using A = F64FArith;
+    const double lam = in.k.d_lamx, hlam = 0.5 * lam, th = in.k.d_theta;
+    const int n = in.nx;
+    if constexpr (WM != WM_NONE) ghost_cells<C>(u, gl, n, in.bc == BC_REFLECT);
+    double fp[3][C], fm[3][C];
+#pragma unroll
+    for (int m = 0; m < C; ++m) {
+        double p[3], q[3], a;
+        flux1<PREC_F64F, double>(u[0][m], u[1][m], u[2][m], in, p, q, a);
+#pragma unroll
+        for (int v = 0; v < 3; ++v) {
+            fp[v][m] = p[v];
+            fm[v][m] = q[v];
+        }
+    }
+    if constexpr (XW > 1) coop_put<double, C, XW>(x, parity, fp, fm);
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+        double FP[C + 4], FM[C + 4], t[C + 1];
+        if constexpr (XW > 1) {
+            double e[4];
+            coop_get<double, XW>(x, parity, v, e);
+            make_window_or<double, C>(fp[v], FP, e[0], e[1]);
+            make_window_or<double, C>(fm[v], FM, e[2], e[3]);
+        } else {
+            make_window<double, C, false>(fp[v], FP);
+            make_window<double, C, false>(fm[v], FM);
+        }
+#pragma unroll
+        for (int f = 0; f <= C; ++f) {
+            const int gf = gl + f;  // face gf lies between cells gf-1 and gf
+            const double G = FP[f + 1] + FM[f + 2];
+            if constexpr (ORDER == 2) {
+                const double d =
+                    slope_at<A, WM, LIM>(FP, f - 1, gf - 1, n, th) - slope_at<A, WM, LIM>(FM, f, gf, n, th);
+                t[f] = __builtin_fma(hlam, d, lam * G);  // lam*(G + d/2)
+            } else {
+                t[f] = lam * G;
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < C; ++m) u[v][m] = u[v][m] + (t[m] - t[m + 1]);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // ns steps of one segment from registers, then the owned cells back to HBM.  Each
 // wall mode gets its own copy of loop and store, so the loop-carried state never
 // has to be moved between the copies' registers.
@@ -338,6 +393,22 @@ __device__ __forceinline__ void advance_c32(float (&hi)[3][C], float (&lo)[3][C]
                 oh[v * P + g] = hi[v][m];
                 ol[v * P + g] = lo[v][m];
             }
+        }
+    }
+}
+
+template <int ORDER, int C, int WM, int LIM, int XW>
+__device__ __forceinline__ void advance_f64f(double (&u)[3][C], const Inst& in, const Item1& it, int gl, int ns,
+                                             const StepArgs& a, const Coop<double, XW>& x) {
+    for (int s = 0; s < ns; ++s) step_f64f<ORDER, C, WM, LIM, XW>(u, in, gl, x, s & 1);
+    double* __restrict__ ou = (double*)a.out0 + in.off;
+    const int64_t P = a.plane;
+#pragma unroll
+    for (int m = 0; m < C; ++m) {
+        const int g = gl + m;
+        if (g >= it.x0 && g < it.x0 + it.w) {
+#pragma unroll
+            for (int v = 0; v < 3; ++v) ou[v * P + g] = u[v][m];
         }
     }
 }
@@ -420,9 +491,15 @@ __global__ __launch_bounds__(64 * WPB) void k_step1d(StepArgs a) {
 #pragma unroll
             for (int v = 0; v < 3; ++v) u[v][m] = ok ? iu[v * P + g] : (RT)(v == 0 ? 1.0 : (v == 1 ? 0.0 : 2.5));
         }
-        if (wm == WM_NONE) advance_ref<PREC, ORDER, C, WM_NONE, LIM, XW>(u, in, it, gl, ns, a, x);
-        else if (ORDER == 1 || wm == WM_GHOST) advance_ref<PREC, ORDER, C, WM_GHOST, LIM, XW>(u, in, it, gl, ns, a, x);
-        else advance_ref<PREC, ORDER, C, WR, LIM, XW>(u, in, it, gl, ns, a, x);
+        if constexpr (PREC == PREC_F64F) {
+            if (wm == WM_NONE) advance_f64f<ORDER, C, WM_NONE, LIM, XW>(u, in, it, gl, ns, a, x);
+            else if (ORDER == 1 || wm == WM_GHOST) advance_f64f<ORDER, C, WM_GHOST, LIM, XW>(u, in, it, gl, ns, a, x);
+            else advance_f64f<ORDER, C, WR, LIM, XW>(u, in, it, gl, ns, a, x);
+        } else {
+            if (wm == WM_NONE) advance_ref<PREC, ORDER, C, WM_NONE, LIM, XW>(u, in, it, gl, ns, a, x);
+            else if (ORDER == 1 || wm == WM_GHOST) advance_ref<PREC, ORDER, C, WM_GHOST, LIM, XW>(u, in, it, gl, ns, a, x);
+            else advance_ref<PREC, ORDER, C, WR, LIM, XW>(u, in, it, gl, ns, a, x);
+        }
     }
 }
 
@@ -472,6 +549,7 @@ hipError_t launch_step1d(int prec, int order, int lim, int C, int wpb, int n_ite
     case PREC_REF32: return launch_p<PREC_REF32>(order, lim, C, wpb, nblocks, s, a);
     case PREC_C32: return launch_p<PREC_C32>(order, lim, C, wpb, nblocks, s, a);
     case PREC_C32K: return launch_p<PREC_C32K>(order, lim, C, wpb, nblocks, s, a);
+    case PREC_F64F: return launch_p<PREC_F64F>(order, lim, C, wpb, nblocks, s, a);
     default: return hipErrorInvalidValue;
     }
 }

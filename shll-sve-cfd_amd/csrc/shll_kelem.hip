@@ -1,4 +1,5 @@
 // shll_kelem.hip -- per-cell kernels around the step loop.
+// F64F stores its state as F64 does, so it shares F64's k_init, k_output and k_sums.
 //   k_init    : primitives -> state      (Compute_U_from_P, base_shll.c:74-85,
 //                                          base_shll_2d.c:124-137)
 //   k_output  : state -> primitives, conserved (Compute_P_from_U + the arrays
@@ -223,17 +224,23 @@ __global__ __launch_bounds__(256) void k_compare(const double* __restrict__ a, i
     block_reduce<4, 6>(s, mx, partial + (int64_t)blockIdx.x * 10);
 }
 
-// Launch counter of the constant-parameter batch graphs: set once before a run of
-// graph chunks (add = 0), advanced by the last node of every chunk (add = 1).
-__global__ __launch_bounds__(64) void k_ctl(int64_t* ctl, int64_t value, int add) {
+// Device counter {j0, anchor, jmax} of the constant-parameter batch graphs: set before a
+// run of graph chunks (add = 0: ctl = {v0, v1, v2}), its launch index advanced by the last
+// node of every chunk (add = 1: ctl[0] += v0).
+__global__ __launch_bounds__(64) void k_ctl(int64_t* ctl, int64_t v0, int64_t v1, int64_t v2, int add) {
     if (threadIdx.x == 0) {
-        const int64_t v = add ? ctl[0] + value : value;
-        ctl[0] = v;
+        if (add) {
+            ctl[0] = ctl[0] + v0;
+        } else {
+            ctl[0] = v0;
+            ctl[1] = v1;
+            ctl[2] = v2;
+        }
     }
 }
 
-hipError_t launch_ctl(hipStream_t st, int64_t* ctl, int64_t value, int add) {
-    hipLaunchKernelGGL(k_ctl, dim3(1), dim3(64), 0, st, ctl, value, add);
+hipError_t launch_ctl(hipStream_t st, int64_t* ctl, int64_t v0, int64_t v1, int64_t v2, int add) {
+    hipLaunchKernelGGL(k_ctl, dim3(1), dim3(64), 0, st, ctl, v0, v1, v2, add);
     return hipGetLastError();
 }
 
@@ -249,7 +256,8 @@ static hipError_t init_p(int dim, dim3 g, hipStream_t st, const Inst* insts, con
 hipError_t launch_init(int prec, int dim, int nblocks, int n_inst, hipStream_t st, const Inst* insts,
                        const double* prim, void* o0, void* o1, int64_t plane) {
     dim3 g(nblocks, n_inst);
-    if (prec == PREC_F64) return init_p<PREC_F64>(dim, g, st, insts, prim, o0, o1, plane);
+    // F64F keeps its state as F64 does (one double plane per variable, same initial values)
+    if (prec == PREC_F64 || prec == PREC_F64F) return init_p<PREC_F64>(dim, g, st, insts, prim, o0, o1, plane);
     if (prec == PREC_REF32) return init_p<PREC_REF32>(dim, g, st, insts, prim, o0, o1, plane);
     return init_p<PREC_C32>(dim, g, st, insts, prim, o0, o1, plane);
 }
@@ -300,8 +308,12 @@ __global__ __launch_bounds__(256) void k_stage(const Inst* insts, const void* s0
 template <int PREC>
 static hipError_t stage_p(int dim, dim3 g, hipStream_t st, const Inst* insts, const void* s0, int64_t plane,
                           double* out) {
-    if (dim == 1) hipLaunchKernelGGL((k_stage<PREC, 1>), g, dim3(256), 0, st, insts, s0, plane, out);
-    else hipLaunchKernelGGL((k_stage<PREC, 2>), g, dim3(256), 0, st, insts, s0, plane, out);
+    if (dim == 1) {
+        hipLaunchKernelGGL((k_stage<PREC, 1>), g, dim3(256), 0, st, insts, s0, plane, out);
+    } else {
+        if constexpr (PREC == PREC_F64F) return hipErrorInvalidValue;  // a 1D form
+        else hipLaunchKernelGGL((k_stage<PREC, 2>), g, dim3(256), 0, st, insts, s0, plane, out);
+    }
     return hipGetLastError();
 }
 
@@ -313,6 +325,7 @@ hipError_t launch_stage(int prec, int dim, int nblocks, hipStream_t st, const In
     case PREC_REF32: return stage_p<PREC_REF32>(dim, g, st, inst, s0, plane, out);
     case PREC_C32: return stage_p<PREC_C32>(dim, g, st, inst, s0, plane, out);
     case PREC_C32K: return stage_p<PREC_C32K>(dim, g, st, inst, s0, plane, out);
+    case PREC_F64F: return stage_p<PREC_F64F>(dim, g, st, inst, s0, plane, out);
     default: return hipErrorInvalidValue;
     }
 }
@@ -328,7 +341,7 @@ static hipError_t output_p(int dim, dim3 g, hipStream_t st, const Inst* insts, c
 hipError_t launch_output(int prec, int dim, int nblocks, int n_inst, hipStream_t st, const Inst* insts,
                          const void* s0, const void* s1, int64_t plane, double* prim, double* cons) {
     dim3 g(nblocks, n_inst);
-    if (prec == PREC_F64) return output_p<PREC_F64>(dim, g, st, insts, s0, s1, plane, prim, cons);
+    if (prec == PREC_F64 || prec == PREC_F64F) return output_p<PREC_F64>(dim, g, st, insts, s0, s1, plane, prim, cons);
     if (prec == PREC_REF32) return output_p<PREC_REF32>(dim, g, st, insts, s0, s1, plane, prim, cons);
     return output_p<PREC_C32>(dim, g, st, insts, s0, s1, plane, prim, cons);
 }
@@ -344,7 +357,7 @@ static hipError_t sums_p(int dim, dim3 g, hipStream_t st, const Inst* insts, con
 hipError_t launch_sums(int prec, int dim, int nblocks, int n_inst, hipStream_t st, const Inst* insts,
                        const void* s0, const void* s1, int64_t plane, double* partial) {
     dim3 g(nblocks, n_inst);
-    if (prec == PREC_F64) return sums_p<PREC_F64>(dim, g, st, insts, s0, s1, plane, partial);
+    if (prec == PREC_F64 || prec == PREC_F64F) return sums_p<PREC_F64>(dim, g, st, insts, s0, s1, plane, partial);
     if (prec == PREC_REF32) return sums_p<PREC_REF32>(dim, g, st, insts, s0, s1, plane, partial);
     return sums_p<PREC_C32>(dim, g, st, insts, s0, s1, plane, partial);
 }

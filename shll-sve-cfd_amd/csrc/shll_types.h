@@ -6,7 +6,7 @@
 
 namespace shll {
 
-enum Prec : int { PREC_F64 = 0, PREC_REF32 = 1, PREC_C32 = 2, PREC_C32K = 3 };
+enum Prec : int { PREC_F64 = 0, PREC_REF32 = 1, PREC_C32 = 2, PREC_C32K = 3, PREC_F64F = 4 };
 enum Bc : int { BC_REFLECT = 0, BC_OUTFLOW = 1 };
 // second-order slope limiters: minmod of the one-sided differences
 // (2nd_order_base_shll.c:268-276) or the MC form minmod(half central difference,
@@ -17,9 +17,10 @@ enum Lim : int { LIM_MINMOD = 0, LIM_MC = 1 };
 //   F64   : d_*  (gamma = 1.4, CV = R/(gamma-1), lam = DT/DX)
 //   REF32 : f_*  (the reference's float constants, base_shll.c:17-22,83-84)
 //   C32   : c_*  (the F64 constants rounded once to float; rcv = 1/CV)
+//   F64F  : d_*, and d_rcv = 1/CV in double
 // theta: the MC limiter's weight (the reference's float alpha = 1.25 in REF32)
 struct Consts {
-    double d_cv, d_gR, d_R, d_lamx, d_lamy, d_theta;
+    double d_cv, d_gR, d_R, d_lamx, d_lamy, d_theta, d_rcv;
     float f_cv, f_gR, f_R, f_lamx, f_lamy, f_theta;
     float c_rcv, c_gR, c_R, c_lamx, c_lamy, c_theta;
 };
@@ -66,13 +67,14 @@ struct StepArgs {
     const void* in1;     //             C32 lo float*
     void* out0;          // state written
     void* out1;
-    // Device-driven launches (constant graph nodes): when jctl is set, this launch's
-    // index is j = *jctl + jofs; it covers n_items = active[j] items (0 if j >= jmax)
-    // and step_base = j * ksteps, overriding the two host-side fields above.
+    // Device-driven launches (constant graph nodes): when jctl is set, it points at the
+    // device counter {j0, anchor, jmax}; this launch's index is j = j0 + jofs, it covers
+    // n_items = active[j] items (0 if j >= jmax) and step_base = anchor + j * ksteps,
+    // overriding the two host-side fields above.  active[] lists the item counts of the
+    // launches of a regular grid of ksteps-step launches that starts at step `anchor`.
     const int64_t* jctl;
     const int32_t* active;
     int32_t jofs;
-    int32_t jmax;
 };
 
 // launch index, item count and step base of a step launch
@@ -84,9 +86,9 @@ struct LaunchView {
 __host__ __device__ inline LaunchView launch_view(const StepArgs& a) {
     LaunchView v{a.n_items, a.step_base};
     if (a.jctl) {
-        const int64_t j = *a.jctl + a.jofs;
-        v.n_items = j < a.jmax ? a.active[j] : 0;
-        v.step_base = j * (int64_t)a.ksteps;
+        const int64_t j = a.jctl[0] + a.jofs;
+        v.n_items = j < a.jctl[2] ? a.active[j] : 0;
+        v.step_base = a.jctl[1] + j * (int64_t)a.ksteps;
     }
     return v;
 }

@@ -21,11 +21,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.dirname(HERE)
 LIB_PATH = os.environ.get("SHLL_AMD_LIB", os.path.join(PKG, "lib", "libshll_amd.so"))
 
-F64, REF32, C32, C32K = 0, 1, 2, 3
+F64, REF32, C32, C32K, F64F = 0, 1, 2, 3, 4
 BC_REFLECT, BC_OUTFLOW = 0, 1
 LIM_MINMOD, LIM_MC = 0, 1
 LIMITER_NAMES = {"minmod": LIM_MINMOD, "mc": LIM_MC}
-PREC_NAMES = {"f64": F64, "ref32": REF32, "c32": C32, "c32k": C32K}
+PREC_NAMES = {"f64": F64, "ref32": REF32, "c32": C32, "c32k": C32K, "f64f": F64F}
 
 OK, EINVAL, ENOMEM, EDEVICE, ENONFINITE, ESTATE = 0, 1, 2, 3, 4, 5
 
@@ -228,11 +228,11 @@ class Solver:
         return out
 
     def state_raw(self):
-        """F64: (nv, N) float64; REF32: (nv, N) float32; C32 / C32K: (hi, lo) float32 pair."""
+        """F64 / F64F: (nv, N) float64; REF32: (nv, N) float32; C32 / C32K: (hi, lo) float32 pair."""
         nb = lib().shll_state_bytes(self.h)
         buf = np.empty(nb, dtype=np.uint8)
         _check(lib().shll_get_state_raw(self.h, buf.ctypes.data_as(ctypes.c_void_p), nb))
-        if self.pb.precision == F64:
+        if self.pb.precision in (F64, F64F):
             return buf.view(np.float64).reshape(self.nv, self.ncell)
         f = buf.view(np.float32)
         if self.pb.precision == REF32:

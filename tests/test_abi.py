@@ -103,3 +103,18 @@ def test_results_reader_reads_the_reference_results_file(tmp_path):
     np.testing.assert_allclose(r["x"], (np.arange(256) + 0.5) / 256, rtol=1e-6)
     with pytest.raises(ValueError):
         S.read_results(path, 255)
+
+
+def test_steps_to_time_reports_a_float_time_loop_that_never_ends(orc):
+    """At 2^24 cells DT = 2^-27 is half an ulp of the float time once time reaches 1/8, so
+    the reference's loop `time += DT` (base_shll.c:200-218) stops growing before t = 0.2
+    and would never end: the library says so (-1) at once instead of spinning."""
+    import time
+    t0 = time.perf_counter()
+    assert S.steps_to_time(S.problem(1, 2, 1 << 24), 0.2) == -1
+    assert time.perf_counter() - t0 < 5.0
+    # the float sum stalls at 1/8: an end time below that still counts normally
+    n = S.steps_to_time(S.problem(1, 2, 1 << 24), 0.1)
+    assert n > 0 and n == orc.steps_to_time(orc.problem(1, 2, 1 << 24), 0.1)
+    # and the largest sizes of the configurations are far from it
+    assert S.steps_to_time(S.problem(1, 2, 65536), 0.2) == 104858

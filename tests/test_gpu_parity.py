@@ -13,7 +13,7 @@ import shll_amd as S
 pytestmark = pytest.mark.gpu
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-MODES = {S.F64: 0, S.REF32: 1, S.C32: 2, S.C32K: 3}  # library precision -> oracle mode
+MODES = {S.F64: 0, S.REF32: 1, S.C32: 2, S.C32K: 3, S.F64F: 4}  # library precision -> oracle mode
 COMP = (S.C32, S.C32K)  # the (hi, lo) compensated forms
 
 
