@@ -1,39 +1,60 @@
 #!/usr/bin/env python3
 """Benchmark of the SHLL time-step loop on one MI355X.
 
-Headline workload (the batched FP32-compensated form the throughput target is set on):
-  4096 independent second-order shock tubes (seed 20261202), N log-uniform in
-  [256, 65536] cells, rho_L/rho_R in [2, 20], T_L/T_R in [0.5, 2], diaphragm in
-  [0.3, 0.7], u = 0, outflow walls, each run to t = 0.2 (its own step count from the
-  reference's float time loop), FP32 fluxes with a compensated (hi, lo) FP32 state
-  updated by one Kahan step per cell and variable (SHLL_C32K), one launch per k steps
-  over an (instance, tile) work list, HIP graphs.
-A "step" is one time step of every unfinished tube.  The default run times the whole
-job (every tube to t = 0.2).  value = total cell-updates / wall time of the timed region.
+Headline workload: 4096 independent second-order shock tubes (seed 20261202), N
+log-uniform in [256, 65536] cells, rho_L/rho_R in [2, 20], T_L/T_R in [0.5, 2], diaphragm
+in [0.3, 0.7], u = 0, outflow walls, each run to t = 0.2 (its own step count from the
+reference's float time loop), one launch per k steps over an (instance, tile) work list,
+HIP graphs.  A "step" is one time step of every unfinished tube; --steps K times exactly
+K of them (the default, 0, times the whole job).  value = cell-updates done / wall time of
+the timed region, inputs resident in HBM.
 
-Also reported (extra): the same batch with the exactly conservative TwoSum update
-(SHLL_C32), the single-instance configurations (1D first order 1024 cells, 1D second
-order 65536 cells and 2D first order 4096^2 in FP64 and both FP32-compensated forms, and
-64 uneven 2D domains), each run in full with its error against the FP64 kernel.
+Timed form (default --form f64f): the fastest form that meets the accuracy target
+L1(rho) <= 1e-6 against the FP64 kernel -- FP64 state and fluxes in the compensated
+kernels' face-term arrangement (SHLL_F64F).  The FP32-compensated forms (--form c32k, c32)
+run ~1.5 times faster but miss that target on second-order tubes of more than ~1000 cells:
+the scheme amplifies round-off about a thousandfold, so FP32 flux rounding grows to
+L1 ~1e-5.  Every form's full run, its accuracy and its rate are in the line (accuracy,
+extra).
 
-cpu_baseline: the oracle's reference-precision port of the reference C loop (unfused
-passes, gcc -O3) on one host core, on a stratified sample of the same tubes.
+accuracy: the timed batch is run to the end outside the timed region and every tube is
+compared with the bitwise FP64 kernel's run of the same batch.  A second FP64 run from
+initial temperatures perturbed by one part in 1e15 tells which tubes FP64 itself cannot
+reproduce to 1e-6: the reference's fixed DT = 0.125*DX lets hot, strongly expanding tubes
+exceed the scheme's stability limit, and there round-off grows without bound; those
+tubes, and the ones that go non-finite in FP64, are counted and reported apart.
 
-cpu_baseline_omp: the same port with one OpenMP worksharing loop per pass (the base-omp
-pattern) on up to 16 host threads.
+roofline: the timed kernel reads and writes HBM once per k steps, so its bound is VALU
+instruction issue.  frac is the measured HBM traffic (rocprofv3 FETCH_SIZE + WRITE_SIZE,
+calibrated by a copy of known size; profiles/) over the event time and the 8 TB/s peak;
+valu.frac is the kernel's VALU instructions per launch (rocprofv3 SQ_INSTS_VALU) over the
+event time and the measured FP64 issue ceiling (profiles/valu_issue_ceiling.md);
+algorithmic_GBps is 48 B per cell-update (read and write the state once) over the event
+time, which temporal blocking lets exceed the peak.
+
+cpu_baseline: the oracle's reference-precision port of the reference loop (unfused
+passes, gcc -O3) on one host core, on a stratified sample of the same tubes (the
+reference has no 1D second-order solver).  cpu_baseline_omp: the same with one OpenMP
+worksharing loop per pass on every host core.  cpu_reference: the reference's own solvers
+compiled from their sources (oracle/_ref), timed over their stage loop only: the base-omp
+second-order solver with OpenMP on 1 core and on every core, and the base-c solvers at the
+configurations' sizes.
 
 --workload domains times the 64 uneven 2D first-order domains instead (same JSON line,
 2D roofline), for profiling the 2D kernel.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extra]
-                       [--workload tubes|domains] [--form c32k|c32] [--launch C,k,waves,graph]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--form f64f|c32k|c32|f64]
+                       [--no-cpu] [--no-extra] [--no-accuracy] [--workload tubes|domains]
+                       [--launch C,k,waves,graph]
 """
 import argparse
 import json
 import os
 import platform
+import re
 import subprocess
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -41,15 +62,27 @@ sys.path.insert(0, os.path.join(ROOT, "shll-sve-cfd_amd"))
 # launch shapes chosen by the library's autotuner on MI355X (tune/make_tune_cache.py);
 # --autotune re-tunes live instead
 TUNE_CACHE = os.path.join(ROOT, "shll-sve-cfd_amd", "tune", "gfx950.txt")
+PROFILES = os.path.join(ROOT, "profiles")
 
 METRIC = "cell-updates/s + % HBM roofline, FP32-comp vs FP64, 1D/2D SHLL, 1×MI355X"
 PEAK_HBM_GBS = 8000.0
 SEED = 20261202
 NTUBES = 4096
+L1_BAR = 1e-6
+PERTURB = 1e-15
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+# form -> (library precision name, description, algorithmic bytes per 1D cell-update, dtype)
+FORMS = {
+    "f64f": ("f64f", "FP64 state and fluxes, compensated kernels' face-term arrangement (SHLL_F64F)", 48, "f64"),
+    "c32k": ("c32k", "FP32 fluxes + (hi,lo) FP32 state, Kahan-compensated update (SHLL_C32K)", 48, "f32"),
+    "c32": ("c32", "FP32 fluxes + (hi,lo) FP32 state, exact TwoSum face update (SHLL_C32)", 48, "f32"),
+    "f64": ("f64", "FP64 state and fluxes, reference association (SHLL_F64, bitwise the FP64 oracle)", 48, "f64"),
+}
 
 
 def bytes_per_update(dim, prec):
@@ -57,24 +90,36 @@ def bytes_per_update(dim, prec):
     (2 * nvar * bytes per value; FP32-compensated stores hi and lo)."""
     import shll_amd as S
     nv = 3 if dim == 1 else 4
-    s = {S.F64: 8, S.REF32: 4, S.C32: 8, S.C32K: 8}[prec]
+    s = {S.F64: 8, S.REF32: 4, S.C32: 8, S.C32K: 8, S.F64F: 8}[prec]
     return 2 * nv * s
 
 
-def traffic_per_update(shape, form="c32k", summary=None):
-    """Calibrated HBM bytes per cell-update of the batch step kernel, from the committed
-    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE scaled by a 1 GiB copy), when the
-    summary was taken with the same launch shape (cells/lane, steps/launch, waves/block);
-    else None."""
-    import re
-    path = os.path.join(ROOT, "profiles", summary or "k_step1d_%s_o2_tubes4096_hbm_traffic.md" % form)
+def profile_figure(summary, key, shape):
+    """A per-cell-update figure ("<key>: <value> (launch shape C,k,w)") from a committed
+    rocprofv3 summary, when it was taken at the same launch shape; else None."""
+    path = os.path.join(PROFILES, summary)
     if not os.path.exists(path):
         return None
-    m = re.search(r"hbm_bytes_per_cell_update:\s*([0-9.eE+-]+)\s*\(launch shape (\d+),(\d+),(\d+)\)",
-                  open(path).read())
+    m = re.search(r"%s:\s*([0-9.eE+-]+)\s*\(launch shape (\d+),(\d+),(\d+)\)" % re.escape(key), open(path).read())
     if not m or tuple(int(x) for x in m.groups()[1:]) != tuple(shape[:3]):
         return None
     return float(m.group(1))
+
+
+def valu_ceiling():
+    """Measured FP32 and FP64 VALU issue ceilings (wavefront instructions per SIMD per ns)
+    and the SIMD count, from profiles/valu_issue_ceiling.md."""
+    path = os.path.join(PROFILES, "valu_issue_ceiling.md")
+    if not os.path.exists(path):
+        return None
+    t = open(path).read()
+    out = {}
+    for key in ("fp32_per_simd_per_ns", "fp64_per_simd_per_ns", "simds", "clock_GHz"):
+        m = re.search(r"%s:\s*([0-9.eE+-]+)" % key, t)
+        if not m:
+            return None
+        out[key] = float(m.group(1))
+    return out
 
 
 def cpu_model():
@@ -86,6 +131,14 @@ def cpu_model():
     except Exception:
         pass
     return platform.processor() or "unknown"
+
+
+def host_cores():
+    """The CPUs this process may use (the box's share of the host)."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
 
 
 def calibration_copy(nbytes):
@@ -120,6 +173,7 @@ def copy_ceiling(nbytes=1 << 30, reps=10):
     hip.hipEventCreate(ctypes.byref(e1))
     hip.hipMemset(a, 1, ctypes.c_size_t(nbytes))
     rates = []
+    rc = 0
     for r in range(reps + 1):
         hip.hipEventRecord(e0, None)
         rc = hip.hipMemcpyAsync(b, a, ctypes.c_size_t(nbytes), 3, None)  # hipMemcpyDeviceToDevice
@@ -141,6 +195,8 @@ def copy_ceiling(nbytes=1 << 30, reps=10):
 
 
 def run_batch(S, pbs, prims, steps, launch, K, W, progress=0, trial=2048):
+    """Build a batch, pick its launch shape, warm up W steps (then reset) and time exactly
+    K steps.  Returns (batch, wall s, average launch ms, launches, event ms)."""
     b = S.Batch(pbs, prims, steps)
     if launch:
         b.set_launch(*launch)
@@ -177,6 +233,90 @@ def updates_done(pbs, steps, K):
     return sum(p.nx * p.ny * min(K, s) for p, s in zip(pbs, steps))
 
 
+# ---------------------------------------------------------------------------
+# accuracy of the tube batch against the bitwise FP64 kernel
+# ---------------------------------------------------------------------------
+class TubeJudge:
+    """The FP64 kernel's run of every tube (bitwise the FP64 oracle), and a second FP64
+    run from initial temperatures perturbed by PERTURB (relative): a tube whose two FP64
+    runs differ by more than the bar cannot be pinned to 1e-6 by FP64 itself."""
+
+    def __init__(self, S, pbs, prims, steps):
+        import numpy as np
+        pf = [S.problem(1, 2, p.nx, precision=S.F64) for p in pbs]
+        t0 = time.perf_counter()
+        self.ref = S.Batch(pf, prims, steps)
+        self.ref.run()
+        self.seconds = time.perf_counter() - t0
+        self.launch = list(self.ref.get_launch())
+        self.bad = self.ref.check_state()
+        pert = []
+        for p in prims:
+            q = p.copy()
+            q[2] = q[2] * (1.0 + PERTURB)
+            pert.append(q)
+        alt = S.Batch(pf, pert, steps)
+        alt.run()
+        bad2 = alt.check_state()
+        self.self_l1 = np.full(len(pbs), np.inf)
+        for k in range(len(pbs)):
+            if self.bad[k] == 0 and bad2[k] == 0:
+                self.self_l1[k] = alt.compare(k, self.ref)["l1_rho"]
+        alt.close()
+        # FP64 pins the tube's answer to a tenth of the bar or better
+        self.reproducible = self.self_l1 <= 0.1 * L1_BAR
+
+    def close(self):
+        self.ref.close()
+
+
+def tube_accuracy(S, b, pbs, judge):
+    """L1 / Linf of rho of every tube of batch b (run to its end) against the FP64 judge."""
+    import numpy as np
+    bad = b.check_state()
+    n = len(pbs)
+    l1 = np.full(n, np.nan)
+    linf = np.full(n, np.nan)
+    for k in range(n):
+        if judge.bad[k] == 0 and bad[k] == 0:
+            e = b.compare(k, judge.ref)
+            l1[k], linf[k] = e["l1_rho"], e["linf_rho"]
+    size = np.array([p.nx for p in pbs])
+    ok = np.isfinite(l1)
+    rep = ok & judge.reproducible
+    bands = {}
+    for lo, hi in ((256, 1024), (1024, 4096), (4096, 16384), (16384, 65537)):
+        m = rep & (size >= lo) & (size < hi)
+        if m.any():
+            bands["N_%d_%d" % (lo, hi - 1 if hi == 65537 else hi)] = {
+                "tubes": int(m.sum()), "l1_rho_median": float(np.median(l1[m])), "l1_rho_max": float(l1[m].max()),
+                "frac_le_1e-6": float((l1[m] <= L1_BAR).mean())}
+    fails = rep & (l1 > L1_BAR)
+    irr = ok & ~judge.reproducible & np.isfinite(judge.self_l1)
+    spread = (l1[irr] / np.maximum(judge.self_l1[irr], 1e-300)) if irr.any() else None
+    return {
+        "judge": "FP64 kernel (bitwise the FP64 oracle), every tube to t = 0.2",
+        "tubes": n,
+        "tubes_nonfinite_fp64": int((judge.bad > 0).sum()),
+        "tubes_nonfinite_this_form_only": int(((bad > 0) & (judge.bad == 0)).sum()),
+        "tubes_fp64_irreproducible": int((~judge.reproducible & (judge.bad == 0)).sum()),
+        "irreproducible_rule": "FP64 from T*(1+1e-15) differs from FP64 by L1(rho) > 1e-7 (a tenth of the bar)",
+        "reproducible_tubes": int(rep.sum()),
+        "l1_rho_median": float(np.median(l1[rep])) if rep.any() else None,
+        "l1_rho_max": float(l1[rep].max()) if rep.any() else None,
+        "linf_rho_max": float(linf[rep].max()) if rep.any() else None,
+        "frac_l1_rho_le_1e-6": float((l1[rep] <= L1_BAR).mean()) if rep.any() else None,
+        "tubes_over_1e-6": int(fails.sum()),
+        "all_finite_tubes": {"count": int(ok.sum()), "l1_rho_median": float(np.median(l1[ok])) if ok.any() else None,
+                             "frac_l1_rho_le_1e-6": float((l1[ok] <= L1_BAR).mean()) if ok.any() else None},
+        "irreproducible_tubes_l1_over_fp64_self_spread_median": float(np.median(spread)) if spread is not None else None,
+        "by_size_reproducible": bands,
+    }
+
+
+# ---------------------------------------------------------------------------
+# CPU baselines
+# ---------------------------------------------------------------------------
 def cpu_baseline(S, pbs, prims, steps, nthreads=1):
     """Oracle REF32 (the reference's arithmetic and pass structure) on every 256th tube of
     the batch for up to 2000 steps each: serially (the base-c loop), or with one OpenMP
@@ -196,19 +336,9 @@ def cpu_baseline(S, pbs, prims, steps, nthreads=1):
     return {"value": total / secs, "unit": "cell-updates/s", "cores": nthreads, "kind": "port",
             "sample": "%d tubes (every 256th of the batch, N=%d..%d), min(steps,2000) steps each, "
                       "%.3g cell-updates in %.1f s; oracle REF32 = reference float arithmetic and "
-                      "unfused pass structure, gcc -O3, %s; host %s" % (
+                      "unfused pass structure, gcc -O3, %s, step loop only; host %s" % (
                           len(idx), min(pbs[k].nx for k in idx), max(pbs[k].nx for k in idx), total, secs,
                           how, cpu_model())}
-
-
-def omp_threads():
-    """Threads for the OpenMP baseline: the CPUs this process may use, at most 16 (a
-    one-GPU share of the host)."""
-    try:
-        n = len(os.sched_getaffinity(0))
-    except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(16, n))
 
 
 def cpu_rate(pb, prim, steps):
@@ -222,32 +352,44 @@ def cpu_rate(pb, prim, steps):
             "sample": "%d steps of the same problem, %.1f s" % (steps, el)}
 
 
-def reference_cpu():
-    """The reference solvers themselves (compiled from their own sources into oracle/_ref by
-    oracle/Makefile, when that build exists), timed on one core at their compiled-in sizes:
-    wall time of the stage loop run by oracle/ref_harness.c for a fixed number of steps."""
-    import tempfile
-    runs = [("base_shll", "base-c/base_shll.c: 1D first order, 256 cells", 256, 20000),
-            ("base_shll_2d", "base-c/base_shll_2d.c: 2D first order, 256^2", 256 * 256, 200),
-            ("2nd_order_base_shll", "base-c/2nd_order_base_shll.c: 2D second order, 256^2", 256 * 256, 100),
-            ("omp_2nd_order_base_shll", "base-omp/2nd_order_base_shll.c (MC limiter, OpenMP off): 2D second "
-             "order, 1024^2", 1024 * 1024, 10)]
+def reference_cpu(cores):
+    """The reference solvers themselves, compiled from their own sources into oracle/_ref by
+    oracle/Makefile (when that build exists), run by ref_harness.c's "time" mode: their
+    initial data and stage sequence, wall time of the step loop only."""
+    runs = [
+        ("omp_on_2nd_order_base_shll", "base-omp/2nd_order_base_shll.c (2D second order, MC limiter, four "
+         "contacts, 1024^2), OpenMP on, gcc-equivalent -O3 (clang)", [(1, 8), (cores, 40)]),
+        ("base_shll_n1024", "base-c/base_shll.c at N = 1024 (configuration 1D Sod, first order), gcc -O3",
+         [(1, 1639)]),
+        ("base_shll_n65536", "base-c/base_shll.c at N = 65536 (first order; the reference has no 1D "
+         "second-order solver), gcc -O3", [(1, 3000)]),
+        ("base_shll_2d_n4096", "base-c/base_shll_2d.c at 4096^2 (the 2D configuration's size, the file's "
+         "implosion data), gcc -O3", [(1, 3)]),
+    ]
     out = {}
-    for exe, what, cells, steps in runs:
+    for exe, what, plan in runs:
         path = os.path.join(ROOT, "oracle", "_ref", exe + ".run")
         if not os.path.exists(path):
             continue
-        with tempfile.TemporaryDirectory() as td:
-            t0 = time.perf_counter()
-            subprocess.run([path, "steps", os.path.join(td, "s"), "-", "0,%d" % steps], cwd=td, check=True,
-                           capture_output=True, timeout=300)
-            dt = time.perf_counter() - t0
-        out[exe] = {"value": cells * steps / dt, "unit": "cell-updates/s", "cores": 1, "kind": "reference",
-                    "sample": "%s, %d steps in %.2f s (process start, set-up and two array dumps included)"
-                              % (what, steps, dt)}
+        for nth, steps in plan:
+            with tempfile.TemporaryDirectory() as td:
+                r = subprocess.run([path, "time", str(steps), str(nth)], cwd=td, capture_output=True, text=True,
+                                   timeout=600)
+            m = re.search(r"cells (\d+) steps (\d+) threads (\d+) seconds ([0-9.eE+-]+)", r.stdout)
+            if r.returncode != 0 or not m:
+                log("reference timing %s failed: %s" % (exe, r.stderr.strip()[-200:]))
+                continue
+            cells, st, th, sec = int(m.group(1)), int(m.group(2)), int(m.group(3)), float(m.group(4))
+            out["%s_%dt" % (exe, th)] = {
+                "value": cells * st / sec, "unit": "cell-updates/s", "cores": th, "kind": "reference",
+                "sample": "%s: %d cells x %d steps in %.2f s on %d thread%s (step loop only); host %s" % (
+                    what, cells, st, sec, th, "" if th == 1 else "s", cpu_model())}
     return out
 
 
+# ---------------------------------------------------------------------------
+# single-instance configurations
+# ---------------------------------------------------------------------------
 def extra_single(S, cpu=True):
     """The single-instance configurations, each in full, with errors against FP64 and (cpu)
     the oracle's one-core rate on a few steps of the same problem."""
@@ -258,20 +400,21 @@ def extra_single(S, cpu=True):
     with S.Solver(pb, ic) as s:
         s.autotune(400)
         s.run_to_time(0.2)
-        s2 = S.Solver(pb, ic)
-        s2.set_launch(*s.get_launch())
+        shape = s.get_launch()
+    with S.Solver(pb, ic) as s2:
+        s2.set_launch(*shape)
         t0 = time.perf_counter()
         n = s2.run_to_time(0.2)
         dt = time.perf_counter() - t0
-        s2.close()
-    out["c0_1d_o1_f64_N1024"] = {"steps": n, "seconds": dt, "cell_updates_per_s": 1024 * n / dt}
+    out["c0_1d_o1_f64_N1024"] = {"steps": n, "seconds": dt, "cell_updates_per_s": 1024 * n / dt,
+                                 "launch": list(shape)}
     if cpu:
         out["c0_1d_o1_f64_N1024"]["cpu_baseline"] = cpu_rate(pb, ic, n)
-    # 1D second order, 65536 cells, t = 0.2, FP64 and both FP32-compensated forms
+    # 1D second order, 65536 cells, t = 0.2: FP64, the fast FP64 form and both compensated forms
     n1 = 65536
     ic = S.ic_tube(n1)
     sol = {}
-    for name, prec in (("f64", S.F64), ("c32", S.C32), ("c32k", S.C32K)):
+    for name, prec in (("f64", S.F64), ("f64f", S.F64F), ("c32", S.C32), ("c32k", S.C32K)):
         pb = S.problem(1, 2, n1, precision=prec)
         s = S.Solver(pb, ic)
         s.autotune(8000)
@@ -286,8 +429,10 @@ def extra_single(S, cpu=True):
         sol[name] = s
         out["c1_1d_o2_%s_N65536" % name] = {"steps": n, "seconds": dt, "cell_updates_per_s": n1 * n / dt,
                                            "launch": list(s.get_launch())}
-    for name in ("c32", "c32k"):
-        out["c1_1d_o2_%s_N65536" % name]["error_vs_f64"] = sol[name].compare(sol["f64"])
+    for name in ("f64f", "c32", "c32k"):
+        e = sol[name].compare(sol["f64"])
+        out["c1_1d_o2_%s_N65536" % name]["error_vs_f64"] = e
+        out["c1_1d_o2_%s_N65536" % name]["meets_l1_1e-6"] = bool(e["l1_rho"] <= L1_BAR)
     if cpu:
         out["c1_1d_o2_f64_N65536"]["cpu_baseline"] = cpu_rate(S.problem(1, 2, n1), ic, 400)
     for s in sol.values():
@@ -314,102 +459,50 @@ def extra_single(S, cpu=True):
                                            "mass_drift": abs(sums1[0] - sums0[0]) / abs(sums0[0]),
                                            "energy_drift": abs(sums1[3] - sums0[3]) / abs(sums0[3])}
     for name in ("c32", "c32k"):
-        out["c2_2d_o1_%s_4096sq" % name]["error_vs_f64"] = sol[name].compare(sol["f64"])
+        e = sol[name].compare(sol["f64"])
+        out["c2_2d_o1_%s_4096sq" % name]["error_vs_f64"] = e
+        out["c2_2d_o1_%s_4096sq" % name]["meets_l1_1e-6"] = bool(e["l1_rho"] <= L1_BAR)
     if cpu:
         out["c2_2d_o1_f64_4096sq"]["cpu_baseline"] = cpu_rate(S.problem(2, 1, n2, n2), ic, 2)
     for s in sol.values():
         s.close()
-    # 64 uneven 2D domains, first order, 1000 steps, both FP32-compensated
-    # forms; the FP64 batch of the same domains judges accuracy
-    pbs, prims = S.domain_batch(4242, 64, precision=S.C32K)
+    # 64 uneven 2D domains, first order, 1000 steps, both FP32-compensated forms; the FP64
+    # batch of the same domains judges accuracy
+    pbs, prims = S.domain_batch(DOMAIN_SEED, NDOMAINS, precision=S.C32K)
     cells = sum(p.nx * p.ny for p in pbs)
     ref = S.Batch([S.problem(2, 1, p.nx, p.ny, precision=S.F64, lx=p.lx, ly=p.ly) for p in pbs], prims,
-                  [1000] * 64)
+                  [DOMAIN_STEPS] * NDOMAINS)
     t1 = time.perf_counter()
     ref.run()
     dt64 = time.perf_counter() - t1
     bad64 = int((ref.check_state() > 0).sum())
     for name, prec in (("c32k", S.C32K), ("c32", S.C32)):
         pbp = [S.problem(2, 1, p.nx, p.ny, precision=prec, lx=p.lx, ly=p.ly) for p in pbs]
-        b = S.Batch(pbp, prims, [1000] * 64)
+        b = S.Batch(pbp, prims, [DOMAIN_STEPS] * NDOMAINS)
         b.advance(10)
         b.reset()
         t0 = time.perf_counter()
         b.run()
         dt = time.perf_counter() - t0
-        errs = [b.compare(k, ref)["l1_rho"] / (pbs[k].lx * pbs[k].ly) for k in range(64)]
+        errs = [b.compare(k, ref)["l1_rho"] / (pbs[k].lx * pbs[k].ly) for k in range(NDOMAINS)]
         launch = list(b.get_launch())
         nonfinite = bad64 + int((b.check_state() > 0).sum())
         b.close()
-        out["c4_2d_o1_%s_64domains" % name] = {"steps": 1000, "cells": cells, "seconds": dt,
+        out["c4_2d_o1_%s_64domains" % name] = {"steps": DOMAIN_STEPS, "cells": cells, "seconds": dt,
                                                "launch": launch,
-                                               "cell_updates_per_s": cells * 1000 / dt,
-                                               "f64_batch_cell_updates_per_s": cells * 1000 / dt64,
+                                               "cell_updates_per_s": cells * DOMAIN_STEPS / dt,
+                                               "f64_batch_cell_updates_per_s": cells * DOMAIN_STEPS / dt64,
                                                "l1_rho_vs_f64_per_unit_area_max": max(errs),
-                                               "l1_rho_vs_f64_per_unit_area_median": sorted(errs)[32],
+                                               "l1_rho_vs_f64_per_unit_area_median": sorted(errs)[NDOMAINS // 2],
+                                               "meets_l1_1e-6": bool(max(errs) <= L1_BAR),
                                                "nonfinite_domains": nonfinite}
     ref.close()
     return out
 
 
-def make_judge(S, pbs, steps, prims, K):
-    """The FP64 kernel run as a batch on every 16th tube in size order (tubes that finish
-    within K steps), the accuracy judge of the FP32-compensated batches."""
-    order = sorted(range(len(pbs)), key=lambda k: pbs[k].nx)
-    sample = [k for k in order[::16] if steps[k] <= K]
-    if not sample:
-        return None
-    ref = S.Batch([S.problem(1, 2, pbs[k].nx, precision=S.F64) for k in sample], [prims[k] for k in sample],
-                  [steps[k] for k in sample])
-    ref.run()
-    return ref, sample, ref.check_state()
-
-
-def tube_errors(S, b, pbs, judge):
-    """Accuracy of an FP32-compensated batch against the FP64 judge.  The reference scheme's
-    fixed time step DT = 0.125*DX assumes wave speeds near R+1; hot, strongly expanding
-    tubes exceed (|u|+a)*DT/DX = 0.5 and the forward-Euler second-order scheme then goes
-    non-finite in FP64 as well, so those tubes are counted and left out of the error
-    statistics.  Tubes on the edge of that instability can blow up in one precision and
-    not the other: the ones non-finite only in FP32-compensated are counted separately and
-    also left out."""
-    import numpy as np
-    if judge is None:
-        return None
-    ref, sample, bad64 = judge
-    bad32 = b.check_state()
-    rows = []
-    c32_only = 0
-    for q, k in enumerate(sample):
-        if bad64[q] > 0:
-            continue
-        if bad32[k] > 0:
-            c32_only += 1
-            continue
-        e = b.compare(k, ref, q)
-        rows.append((pbs[k].nx, e["l1_rho"], e["linf_rho"]))
-    l1 = np.array([r[1] for r in rows])
-    n = np.array([r[0] for r in rows])
-    bands = {}
-    for lo, hi in ((256, 1024), (1024, 4096), (4096, 16384), (16384, 65537)):
-        m = (n >= lo) & (n < hi)
-        if m.any():
-            bands["N_%d_%d" % (lo, hi - 1 if hi == 65537 else hi)] = {
-                "tubes": int(m.sum()), "l1_rho_median": float(np.median(l1[m])), "l1_rho_max": float(l1[m].max()),
-                "frac_le_1e-6": float((l1[m] <= 1e-6).mean())}
-    return {
-        "judge": "FP64 kernel, every 16th tube in size order (%d tubes)" % len(sample),
-        "tubes_nonfinite_fp64_in_sample": int((bad64 > 0).sum()),
-        "tubes_nonfinite_c32_only_in_sample": c32_only,
-        "tubes_nonfinite_c32_all": int((bad32 > 0).sum()),
-        "compared_tubes": len(rows),
-        "l1_rho_median": float(np.median(l1)) if len(l1) else None,
-        "l1_rho_max": float(l1.max()) if len(l1) else None,
-        "frac_l1_rho_le_1e-6": float((l1 <= 1e-6).mean()) if len(l1) else None,
-        "by_size": bands,
-    }
-
-
+# ---------------------------------------------------------------------------
+# 2D domains workload
+# ---------------------------------------------------------------------------
 DOMAIN_SEED = 4242
 NDOMAINS = 64
 DOMAIN_STEPS = 1000
@@ -435,63 +528,96 @@ def cpu_baseline_2d(pbs, prims, nthreads=1, steps=40):
                           len(idx), steps, total, secs, how, cpu_model())}
 
 
+def roofline(kind, upd, launches, ev_ms, avg_ms, B, shape, traffic_summary, valu_summary, kernel, prec_dtype):
+    """Roofline object of the timed step kernel: measured HBM traffic against the 8 TB/s
+    peak (frac), the VALU issue fraction, and the algorithmic bytes rate."""
+    t = ev_ms * 1e-3
+    per_launch = upd / max(1, launches)
+    tpu = profile_figure(traffic_summary, "hbm_bytes_per_cell_update", shape)
+    vpu = profile_figure(valu_summary, "valu_insts_per_cell_update", shape)
+    ceil = valu_ceiling()
+    traffic = tpu * per_launch if tpu is not None else None
+    achieved = tpu * upd / t / 1e9 if (tpu is not None and t > 0) else None
+    valu = None
+    if vpu is not None and ceil is not None and t > 0:
+        per_ns = ceil["fp64_per_simd_per_ns"] if prec_dtype == "f64" else ceil["fp32_per_simd_per_ns"]
+        rate = vpu * upd / t  # wavefront VALU instructions per second
+        valu = {"achieved_wave_insts_per_s": rate, "peak_wave_insts_per_s": per_ns * 1e9 * ceil["simds"],
+                "frac": rate / (per_ns * 1e9 * ceil["simds"]), "insts_per_cell_update": vpu,
+                "ceiling": "%s fma issue, measured (profiles/valu_issue_ceiling.md, %.4g per SIMD per ns, "
+                           "%d SIMDs, max clock %.2f GHz)" % ("FP64" if prec_dtype == "f64" else "FP32", per_ns,
+                                                             ceil["simds"], ceil["clock_GHz"])}
+    k = shape[1]
+    return {
+        "bound": "valu" if k > 1 else "hbm",
+        "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+        "frac": achieved / PEAK_HBM_GBS if achieved is not None else None,
+        "traffic": traffic,
+        "valu": valu,
+        "algorithmic_GBps": B * upd / t / 1e9 if t > 0 else None,
+        "algorithmic_bytes_per_cell_update": B,
+        "kernel": kernel,
+        "avg_launch_ms": avg_ms,
+        "measured_copy_GBps": copy_ceiling(),
+        "note": "k = %d steps per launch read and write HBM once: the kernel is bound by VALU issue. achieved = "
+                "measured HBM bytes (rocprofv3 FETCH_SIZE + WRITE_SIZE per cell-update at this launch shape, "
+                "profiles/%s) x cell-updates / HIP-event time; traffic = those bytes per launch; "
+                "algorithmic_GBps = %d B per cell-update (state read and written once) / event time; valu = "
+                "SQ_INSTS_VALU per cell-update (profiles/%s) x cell-updates / event time against the measured "
+                "issue ceiling" % (k, traffic_summary, B, valu_summary),
+    }
+
+
 def main_domains(S, args):
     """64 uneven first-order 2D domains (side lengths log-uniform in [256, 2048], 1-4
     circles each), 1000 steps, FP32-compensated, one launch per k steps over the
     (domain, tile) work list; the FP64 batch of the same domains judges accuracy."""
-    form = S.PREC_NAMES[args.form]
-    pbs, prims = S.domain_batch(DOMAIN_SEED, NDOMAINS, precision=form)
+    form = args.form if args.form in ("c32k", "c32", "f64") else "c32k"
+    prec = S.PREC_NAMES[form]
+    pbs, prims = S.domain_batch(DOMAIN_SEED, NDOMAINS, precision=prec)
     steps = [DOMAIN_STEPS] * NDOMAINS
     K = args.steps if args.steps > 0 else DOMAIN_STEPS
     launch = tuple(int(x) for x in args.launch.split(",")) if args.launch else None
     b, wall, avg_ms, launches, ev_ms = run_batch(S, pbs, prims, steps, launch, K, args.warmup, args.progress, 256)
     upd = updates_done(pbs, steps, K)
     lau = list(b.get_launch())
-    B = bytes_per_update(2, form)
-    achieved = B * upd / (ev_ms * 1e-3) / 1e9 if ev_ms > 0 else 0.0
+    B = bytes_per_update(2, prec)
     log("timed %d steps: %.3e cell-updates in %.3f s (events %.1f ms, %d launches)" % (K, upd, wall, ev_ms, launches))
     acc = None
     if not args.no_accuracy:
-        ref = S.Batch([S.problem(2, 1, p.nx, p.ny, precision=S.F64, lx=p.lx, ly=p.ly) for p in pbs], prims,
-                      [K] * NDOMAINS)
+        if K < DOMAIN_STEPS:
+            b.run()
+        ref = S.Batch([S.problem(2, 1, p.nx, p.ny, precision=S.F64, lx=p.lx, ly=p.ly) for p in pbs], prims, steps)
         t0 = time.perf_counter()
         ref.run()
         dt64 = time.perf_counter() - t0
         errs = [b.compare(k, ref)["l1_rho"] / (pbs[k].lx * pbs[k].ly) for k in range(NDOMAINS)]
-        acc = {"judge": "FP64 batch of the same domains (its launch shape %s, %.3e cell-updates/s)" % (
-                   list(ref.get_launch()), upd / dt64),
+        acc = {"judge": "FP64 batch of the same domains, 1000 steps (its launch shape %s, %.3e cell-updates/s)" % (
+                   list(ref.get_launch()), updates_done(pbs, steps, DOMAIN_STEPS) / dt64),
                "l1_rho_per_unit_area_max": max(errs), "l1_rho_per_unit_area_median": sorted(errs)[NDOMAINS // 2],
+               "meets_l1_1e-6": bool(max(errs) <= L1_BAR),
                "nonfinite_domains": int((b.check_state() > 0).sum() + (ref.check_state() > 0).sum())}
         ref.close()
     b.close()
-    tpu = traffic_per_update(lau, args.form, "k_step2d_tb_%s_domains64_hbm_traffic.md" % args.form)
-    tr = tpu * upd / launches if tpu is not None and launches > 0 else None
-    names = {S.C32: "exact TwoSum face update", S.C32K: "Kahan-compensated update"}
+    kern = ("shll::k_step2d_tb<%d, %d, %d> (first order, %d steps per launch)" % (prec, lau[1], lau[2], lau[1])
+            if lau[1] > 1 else "shll::k_step2d<%d, 1, %d, 0>" % (prec, lau[2]))
     result = {
         "metric": METRIC, "value": upd / wall, "unit": "cell-updates/s", "n_gpus": 1, "steps": K,
         "warmup": args.warmup, "ms_per_step": wall * 1e3 / K, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "batched 2D first-order SHLL domains, FP32 fluxes + (hi,lo) FP32 state, " + names[form],
+        "vs_baseline": None, "dtype": FORMS[form][3], "data": "synthetic",
+        "config": {"workload": "batched 2D first-order SHLL domains, " + FORMS[form][1],
                    "domains": NDOMAINS, "seed": DOMAIN_SEED, "side_range": [256, 2048],
                    "cells": int(sum(p.nx * p.ny for p in pbs)), "cell_updates_timed": int(upd),
                    "launches": launches, "parallelism": "1 GPU",
                    "launch_shape": {"steps_per_launch": lau[1], "waves_per_block": lau[2], "graph_steps": lau[3]}},
-        "roofline": {
-            "bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-            "frac": achieved / PEAK_HBM_GBS, "traffic": tr,
-            "kernel": "shll::k_step2d_tb<%d, %d, %d> (FP32-compensated, first order, %d steps per launch)" % (
-                form, lau[1], lau[2], lau[1]) if lau[1] > 1 else "shll::k_step2d<%d, 1, %d, 0>" % (form, lau[2]),
-            "algorithmic_bytes_per_cell_update": B, "avg_launch_ms": avg_ms,
-            "measured_copy_GBps": copy_ceiling(),
-            "note": "achieved = 64 B x cell-updates / HIP-event time of the launches (k steps per launch read "
-                    "and write the state once). traffic = calibrated rocprofv3 FETCH_SIZE+WRITE_SIZE per "
-                    "cell-update (profiles/k_step2d_tb_%s_domains64_hbm_traffic.md) x cell-updates per launch"
-                    % args.form},
+        "roofline": roofline("2d", upd, launches, ev_ms, avg_ms, B, lau,
+                             "k_step2d_tb_%s_domains64_hbm_traffic.md" % form,
+                             "k_step2d_tb_%s_domains64_sq_counters.md" % form, kern, FORMS[form][3]),
         "accuracy": acc,
     }
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline_2d(pbs, prims)
-        result["cpu_baseline_omp"] = cpu_baseline_2d(pbs, prims, nthreads=omp_threads())
+        result["cpu_baseline_omp"] = cpu_baseline_2d(pbs, prims, nthreads=host_cores())
     else:
         result["cpu_baseline"] = None
     print(json.dumps(result))
@@ -508,11 +634,12 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true", help="skip the FP64 judge of the batch")
     ap.add_argument("--progress", type=int, default=0,
                     help="advance the timed run in pieces of this many steps and log each (profiling runs)")
-    ap.add_argument("--launch", default="", help="C,k,waves,graph (default: autotune)")
+    ap.add_argument("--launch", default="", help="C,k,waves,graph (default: the recorded autotuner choice)")
     ap.add_argument("--calib-copy", type=int, default=0, help="bytes of a device copy to run first (profiling)")
     ap.add_argument("--autotune", action="store_true", help="re-run the autotuner instead of its recorded choices")
-    ap.add_argument("--form", default="c32k", choices=["c32k", "c32"],
-                    help="compensated update of the timed batch: Kahan (c32k) or exact TwoSum (c32)")
+    ap.add_argument("--form", default="f64f", choices=sorted(FORMS),
+                    help="arithmetic of the timed batch: f64f (FP64 face form, meets L1 1e-6; default), c32k / c32 "
+                         "(FP32-compensated), f64 (bitwise FP64)")
     ap.add_argument("--workload", default="tubes", choices=["tubes", "domains"],
                     help="tubes: the 4096-tube second-order batch (headline); domains: 64 uneven 2D "
                          "first-order domains, 1000 steps")
@@ -529,49 +656,55 @@ def main():
     if args.workload == "domains":
         main_domains(S, args)
         return
-    form = S.PREC_NAMES[args.form]
-    other = S.C32 if form == S.C32K else S.C32K
-    names = {S.C32: "FP32 fluxes + (hi,lo) FP32 state, exact TwoSum face update",
-             S.C32K: "FP32 fluxes + (hi,lo) FP32 state, Kahan-compensated update"}
+    prec = S.PREC_NAMES[args.form]
     t0 = time.perf_counter()
-    pbs, prims, steps = S.tube_batch(SEED, NTUBES, precision=form)
+    pbs, prims, steps = S.tube_batch(SEED, NTUBES, precision=prec)
+    total_steps = max(steps)
     log("generated %d tubes in %.2fs, %d cells, max steps %d" % (len(pbs), time.perf_counter() - t0,
-                                                                 sum(p.nx for p in pbs), max(steps)))
-    K = args.steps if args.steps > 0 else max(steps)
+                                                                 sum(p.nx for p in pbs), total_steps))
+    K = min(args.steps, total_steps) if args.steps > 0 else total_steps
     launch = tuple(int(x) for x in args.launch.split(",")) if args.launch else None
     b, wall, avg_ms, launches, ev_ms = run_batch(S, pbs, prims, steps, launch, K, args.warmup, args.progress)
     upd = updates_done(pbs, steps, K)
     value = upd / wall
-    B = bytes_per_update(1, form)
-    achieved = B * upd / (ev_ms * 1e-3) / 1e9 if ev_ms > 0 else 0.0
+    B = bytes_per_update(1, prec)
     lau = list(b.get_launch())
     log("timed %d steps: %.3e cell-updates in %.3f s (events %.1f ms, %d launches)" % (K, upd, wall, ev_ms, launches))
-    judge = None if args.no_accuracy else make_judge(S, pbs, steps, prims, K)
-    errs = tube_errors(S, b, pbs, judge)
+    judge = acc = None
+    forms_out = {}
+    if not args.no_accuracy:
+        t1 = time.perf_counter()
+        judge = TubeJudge(S, pbs, prims, steps)
+        log("FP64 judge and its perturbed twin in %.1f s" % (time.perf_counter() - t1))
+        if K < total_steps:
+            b.run()  # the rest of the timed batch, outside the timed region
+        acc = tube_accuracy(S, b, pbs, judge)
+        acc["timed_steps_then_run_to_end"] = K < total_steps
     b.close()
-    alt = f64b = None
     if not args.no_extra:
-        # the same batch with the other compensated update, its own tuned shape
-        pbo = [S.problem(1, 2, p.nx, precision=other) for p in pbs]
-        bo, wo, ao, lo_, eo = run_batch(S, pbo, prims, steps, None, K, args.warmup, args.progress)
-        alt = {"form": names[other], "precision": "c32" if other == S.C32 else "c32k",
-               "cell_updates_per_s": upd / wo, "avg_launch_ms": ao, "launches": lo_,
-               "launch_shape": list(bo.get_launch()), "accuracy": tube_errors(S, bo, pbo, judge)}
-        log("other form: %.3e cell-updates/s" % (upd / wo))
-        bo.close()
-        # the same batch in FP64 (state and fluxes): the precision that meets any error bar
-        # against the FP64 oracle (it is bitwise that oracle), at its own tuned shape
-        pbf = [S.problem(1, 2, p.nx, precision=S.F64) for p in pbs]
-        bf, wf, af, lf, _ = run_batch(S, pbf, prims, steps, None, K, args.warmup, args.progress)
-        f64b = {"form": "FP64 state and fluxes, reference association (bitwise the FP64 CPU oracle)",
-                "cell_updates_per_s": upd / wf, "avg_launch_ms": af, "launches": lf,
-                "launch_shape": list(bf.get_launch())}
-        log("FP64 batch: %.3e cell-updates/s" % (upd / wf))
-        bf.close()
+        # every other form of the same batch: full run at its recorded shape, rate and accuracy
+        for name in ("f64f", "c32k", "c32", "f64"):
+            if name == args.form:
+                continue
+            if name == "f64" and judge is not None:
+                forms_out[name] = {"form": FORMS[name][1], "cell_updates_per_s": updates_done(pbs, steps, total_steps)
+                                   / judge.seconds, "launch_shape": judge.launch,
+                                   "accuracy": "bitwise the FP64 oracle (the judge)"}
+                continue
+            pp = [S.problem(1, 2, p.nx, precision=S.PREC_NAMES[name]) for p in pbs]
+            bo, wo, ao, lo_, _ = run_batch(S, pp, prims, steps, None, total_steps, args.warmup)
+            forms_out[name] = {"form": FORMS[name][1], "cell_updates_per_s": updates_done(pbs, steps, total_steps) / wo,
+                               "avg_launch_ms": ao, "launches": lo_, "launch_shape": list(bo.get_launch())}
+            if judge is not None and name != "f64":
+                a = tube_accuracy(S, bo, pbs, judge)
+                forms_out[name]["accuracy"] = a
+                forms_out[name]["meets_l1_1e-6_on_reproducible_tubes"] = a["tubes_over_1e-6"] == 0
+            log("%s: %.3e cell-updates/s" % (name, forms_out[name]["cell_updates_per_s"]))
+            bo.close()
     if judge is not None:
-        judge[0].close()
-    tpu = traffic_per_update(lau, args.form)
-    tr = tpu * upd / launches if tpu is not None and launches > 0 else None
+        judge.close()
+    kern = "shll::k_step1d<%d, 2, %d, %d, 0> (%s, second order minmod, %d cells/lane, %d waves/block)" % (
+        prec, lau[0], lau[2], FORMS[args.form][1], lau[0], lau[2])
     result = {
         "metric": METRIC,
         "value": value,
@@ -583,45 +716,31 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": FORMS[args.form][3],
         "data": "synthetic",
         "config": {
-            "workload": "batched 1D second-order SHLL shock tubes, " + names[form],
+            "workload": "batched 1D second-order SHLL shock tubes, " + FORMS[args.form][1],
             "tubes": NTUBES, "seed": SEED, "cells": int(sum(p.nx for p in pbs)),
-            "N_range": [256, 65536], "t_end": 0.2, "max_steps": int(max(steps)),
+            "N_range": [256, 65536], "t_end": 0.2, "max_steps": int(total_steps),
             "cell_updates_timed": int(upd), "launches": launches, "parallelism": "1 GPU",
             "launch_shape": {"cells_per_lane": lau[0], "steps_per_launch": lau[1], "waves_per_block": lau[2],
                              "graph_steps": lau[3]},
         },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": PEAK_HBM_GBS,
-            "unit": "GB/s",
-            "frac": achieved / PEAK_HBM_GBS,
-            "traffic": tr,
-            "kernel": "shll::k_step1d<%d, 2, %d, %d, 0> (FP32-compensated %s, second order minmod, %d cells/lane, %d waves/block)" % (
-                form, lau[0], lau[2], "Kahan" if form == S.C32K else "TwoSum", lau[0], lau[2]),
-            "algorithmic_bytes_per_cell_update": B,
-            "avg_launch_ms": avg_ms,
-            "measured_copy_GBps": copy_ceiling(),
-            "note": "achieved = 48 B x cell-updates / HIP-event time of the launches; temporal blocking "
-                    "(k steps per launch from registers) lets it exceed the HBM peak. traffic = HBM bytes per "
-                    "launch of the timed run: calibrated rocprofv3 FETCH_SIZE+WRITE_SIZE per cell-update "
-                    "(profiles/k_step1d_%s_o2_tubes4096_hbm_traffic.md) x cell-updates per launch" % args.form,
-        },
-        "accuracy": errs,
+        "roofline": roofline("1d", upd, launches, ev_ms, avg_ms, B, lau,
+                             "k_step1d_%s_o2_tubes4096_hbm_traffic.md" % args.form,
+                             "k_step1d_%s_o2_tubes4096_sq_counters.md" % args.form, kern, FORMS[args.form][3]),
+        "accuracy": acc,
     }
     if not args.no_cpu:
+        cores = host_cores()
         result["cpu_baseline"] = cpu_baseline(S, pbs, prims, steps)
-        result["cpu_baseline_omp"] = cpu_baseline(S, pbs, prims, steps, nthreads=omp_threads())
+        result["cpu_baseline_omp"] = cpu_baseline(S, pbs, prims, steps, nthreads=cores)
+        result["cpu_reference"] = reference_cpu(cores)
     else:
         result["cpu_baseline"] = None
     if not args.no_extra:
-        result["extra"] = {"c3_other_update": alt, "c3_f64_batch": f64b}
+        result["extra"] = {"c3_forms": forms_out}
         result["extra"].update(extra_single(S, cpu=not args.no_cpu))
-        if not args.no_cpu:
-            result["extra"]["cpu_reference_solvers"] = reference_cpu()
     print(json.dumps(result))
 
 

@@ -22,11 +22,21 @@
  *   <bin> main  <out.bin>
  *   <bin> steps <out_prefix> <ic.bin|-> <k1,k2,...>   -> <out_prefix>_<k>.bin
  *   ic.bin: float prim[nvar][ncells] replacing the reference initial data.
+ *   <bin> time  <nsteps> <nthreads>
+ *       CPU baseline: the reference's own initial data and stage sequence for nsteps
+ *       steps, timed with CLOCK_MONOTONIC around the step loop only (set-up, initial
+ *       U <- P and the output are outside).  In the OpenMP build (REF_TIMED_OMP, the
+ *       base-omp file compiled with OpenMP on) the loop runs as base-omp's main() runs
+ *       it (base-omp/2nd_order_base_shll.c:620-653): every thread of one parallel
+ *       region calls each stage, whose orphaned `omp for` loops share the cells, with a
+ *       barrier per step.  Other builds are serial (nthreads must be 1).
+ *       Prints: cells <N> steps <nsteps> threads <nthreads> seconds <t> checksum <sum of rho>
  */
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #if defined(REF_O1_1D)
 #define NVAR 3
@@ -108,8 +118,64 @@ static void load_ic(const char *path)
     fclose(f);
 }
 
+static double now_s(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+/* the timed step loop of the "time" mode */
+static int time_steps(long nsteps, int nthreads)
+{
+    double t0 = 0.0, t1 = 0.0, sum = 0.0;
+    long i;
+    float *P[4], *U[4];
+    Allocate_and_Init_Memory();
+#if defined(REF_TIMED_OMP)
+#pragma omp parallel num_threads(nthreads)
+    {
+        long s;
+        Compute_U_from_P();
+        Compute_P_from_U();
+#pragma omp barrier
+#pragma omp master
+        t0 = now_s();
+        for (s = 0; s < nsteps; s++) {
+            Compute_F_from_P();
+            Update_U_from_F();
+            Compute_P_from_U();
+#pragma omp barrier
+        }
+#pragma omp master
+        t1 = now_s();
+    }
+#else
+    long s;
+    if (nthreads != 1) {
+        fprintf(stderr, "this reference build is serial: use 1 thread\n");
+        return 1;
+    }
+    Compute_U_from_P();
+    Compute_P_from_U();
+    t0 = now_s();
+    for (s = 0; s < nsteps; s++) {
+        Compute_F_from_P();
+        Update_U_from_F();
+        Compute_P_from_U();
+    }
+    t1 = now_s();
+#endif
+    arrays(P, U);
+    for (i = 0; i < N; i++) sum += P[0][i];
+    printf("cells %d steps %ld threads %d seconds %.6f checksum %.9e\n", N, nsteps, nthreads, t1 - t0, sum);
+    Free_Memory();
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
+    if (argc == 4 && strcmp(argv[1], "time") == 0) return time_steps(atol(argv[2]), atoi(argv[3]));
     if (argc == 3 && strcmp(argv[1], "main") == 0) {
         g_dump_on_free = argv[2];
         reference_main();

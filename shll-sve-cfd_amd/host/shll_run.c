@@ -13,10 +13,12 @@
  * The reference's compile-time constants become flags.  Defaults reproduce
  * base_shll.c (1D, N=256, t=0.2, Sod, reflective walls, reference float arithmetic).
  *
- *   shll_run [--dim 1|2] [--order 1|2] [--nx N] [--ny N] [--precision f64|ref32|c32|c32k]
+ *   shll_run [--dim 1|2] [--order 1|2] [--nx N] [--ny N] [--precision f64|ref32|c32|c32k|f64f]
  *            [--bc reflect|outflow] [--ic sod|implosion|four-shocks|four-contacts|circle]
  *            [--limiter minmod|mc] [--theta A] [--t-end T] [--steps K] [--out results.dat]
- *            [--full] [--sums]
+ *            [--full] [--sums] [--dump PATH]
+ * --dump writes the end state in binary: the primitives (nvar*nx*ny doubles, SoA) followed by
+ * the state in its storage format (shll_get_state_raw).
  * --limiter mc (theta 1.25) with --dim 2 --order 2 --nx 1024 is base-omp/2nd_order_base_shll.c:
  * its default initial data are the four contacts and its TOTAL_TIME 0.3 (:43).
  */
@@ -29,16 +31,16 @@
 static void usage(void)
 {
     fprintf(stderr,
-            "usage: shll_run [--dim 1|2] [--order 1|2] [--nx N] [--ny N] [--precision f64|ref32|c32|c32k]\n"
+            "usage: shll_run [--dim 1|2] [--order 1|2] [--nx N] [--ny N] [--precision f64|ref32|c32|c32k|f64f]\n"
             "                [--bc reflect|outflow] [--ic sod|implosion|four-shocks|four-contacts|circle]\n"
             "                [--limiter minmod|mc] [--theta A] [--t-end T] [--steps K] [--out PATH]\n"
-            "                [--full] [--sums]\n");
+            "                [--full] [--sums] [--dump PATH]\n");
 }
 
 int main(int argc, char **argv)
 {
     shll_problem pb;
-    const char *ic = NULL, *out = "results.dat";
+    const char *ic = NULL, *out = "results.dat", *dump = NULL;
     float t_end = -1.0f;
     long long steps = -1;
     int sums = 0, full = 0, i, order_set = 0, bc_set = 0, nv;
@@ -64,6 +66,7 @@ int main(int argc, char **argv)
             pb.precision = !strcmp(v, "f64")    ? SHLL_F64
                            : !strcmp(v, "c32")  ? SHLL_C32
                            : !strcmp(v, "c32k") ? SHLL_C32K
+                           : !strcmp(v, "f64f") ? SHLL_F64F
                                                 : SHLL_REF32;
         else if (!strcmp(a, "--bc")) { pb.bc = !strcmp(v, "outflow") ? SHLL_BC_OUTFLOW : SHLL_BC_REFLECT; bc_set = 1; }
         else if (!strcmp(a, "--ic")) ic = v;
@@ -72,6 +75,7 @@ int main(int argc, char **argv)
         else if (!strcmp(a, "--t-end")) t_end = (float)atof(v);
         else if (!strcmp(a, "--steps")) steps = atoll(v);
         else if (!strcmp(a, "--out")) out = v;
+        else if (!strcmp(a, "--dump")) dump = v;
         else { usage(); return 2; }
         i++;
     }
@@ -111,6 +115,25 @@ int main(int argc, char **argv)
             printf("sums:");
             for (n = 0; n < nv; n++) printf(" %.17g", s[n]);
             printf("\n");
+        }
+    }
+    if (dump) {
+        size_t ncell = (size_t)(pb.nx * pb.ny), raw = shll_state_bytes(ctx);
+        double *p = (double *)malloc(sizeof(double) * (size_t)nv * ncell);
+        void *r = malloc(raw);
+        FILE *f = fopen(dump, "wb");
+        st = (p && r && f) ? shll_get_primitives(ctx, p) : SHLL_ENOMEM;
+        if (st == SHLL_OK) st = shll_get_state_raw(ctx, r, raw);
+        if (st == SHLL_OK && (fwrite(p, sizeof(double), (size_t)nv * ncell, f) != (size_t)nv * ncell ||
+                              fwrite(r, 1, raw, f) != raw))
+            st = SHLL_EINVAL;
+        if (f && fclose(f) != 0) st = SHLL_EINVAL;
+        free(p);
+        free(r);
+        if (st != SHLL_OK) {
+            fprintf(stderr, "dump to %s failed: %s\n", dump, shll_last_error());
+            shll_free(ctx);
+            return 1;
         }
     }
     if (out && strcmp(out, "-") != 0) {
