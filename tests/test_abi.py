@@ -118,3 +118,17 @@ def test_steps_to_time_reports_a_float_time_loop_that_never_ends(orc):
     assert n > 0 and n == orc.steps_to_time(orc.problem(1, 2, 1 << 24), 0.1)
     # and the largest sizes of the configurations are far from it
     assert S.steps_to_time(S.problem(1, 2, 65536), 0.2) == 104858
+
+
+def test_integration_example_compiles_against_the_header(tmp_path):
+    """INTEGRATION.md's gpu_main.c (its first C block) is valid C99 against include/shll.h
+    (the GPU test links and runs it with the reference's own base_shll.c)."""
+    import subprocess
+    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    m = re.search(r"^```c\n(.*?)^```$", text, flags=re.S | re.M)
+    assert m and "int main(void)" in m.group(1)
+    src = tmp_path / "gpu_main.c"
+    src.write_text(m.group(1))
+    r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-fsyntax-only", "-I", os.path.join(ROOT, "include"),
+                        str(src)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

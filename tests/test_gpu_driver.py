@@ -105,3 +105,17 @@ def test_momentum_change_equals_wall_impulse_2d():
     assert abs(d[2] - iy) <= 1e-13 * (np.abs(c0[2]) + abs(iy) + 1.0), (d[2], iy)
     assert abs(d[0]) <= 1e-13 * c0[0] and abs(d[3]) <= 1e-13 * c0[3]
     assert abs(ix) > 1e-6 and abs(iy) > 1e-6
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(ROOT, "oracle", "_ref", "integration_base_shll.run")),
+                    reason="built only where the reference tree was present at build time")
+def test_integration_example_is_the_reference_file_on_the_gpu(tmp_path):
+    """INTEGRATION.md's gpu_main.c linked with the reference's own base_shll.c (main renamed):
+    the reference's stdout line and results.dat, byte for byte."""
+    g = np.load(os.path.join(GOLD, "o1_1d_main.npz"), allow_pickle=False)
+    exe = os.path.join(ROOT, "oracle", "_ref", "integration_base_shll.run")
+    print("[gpu-case] integration example", flush=True)
+    r = subprocess.run([exe], cwd=tmp_path, capture_output=True, timeout=120)
+    assert r.returncode == 0, r.stderr.decode()
+    assert r.stdout == bytes(g["stdout"])
+    assert (tmp_path / "results.dat").read_bytes() == bytes(g["results_dat"])
