@@ -442,11 +442,13 @@ __global__ __launch_bounds__(64 * WPB) void k_step1d(StepArgs a) {
     constexpr int XW = WPB > 1 && C >= 2 ? WPB : 1;
     __shared__ __align__(16) RT xlds[XW > 1 ? 2 * XW * CoopSlots<RT>::WAVE : 1];
     const int wib = uniform((int)threadIdx.x >> 6);
-    const int item = XW > 1 ? (int)blockIdx.x : (int)blockIdx.x * WPB + wib;
     const LaunchView lv = launch_view(a);
+    constexpr int IPB = XW > 1 ? 1 : WPB;  // items per block
+    const int blk = xcd_work((int)blockIdx.x, (lv.n_items + IPB - 1) / IPB);
+    const int item = XW > 1 ? blk : blk * WPB + wib;
     // every wavefront of a cooperative block has the same item and step count, so all of
     // them return here together or reach the same number of barriers
-    if (item >= lv.n_items) return;
+    if (blk < 0 || item >= lv.n_items) return;
     const int lane = (int)threadIdx.x & 63;
     const Item1 it = ((const Item1*)a.items)[item];
     const Inst in = a.inst[it.inst];
@@ -542,7 +544,7 @@ int step1d_items_per_block(int C, int wpb) { return wpb > 1 && C >= 2 ? 1 : wpb;
 hipError_t launch_step1d(int prec, int order, int lim, int C, int wpb, int n_items, hipStream_t s,
                          const StepArgs& a) {
     const int ipb = step1d_items_per_block(C, wpb);
-    const int nblocks = (n_items + ipb - 1) / ipb;
+    const int nblocks = xcd_grid((n_items + ipb - 1) / ipb);
     if (nblocks <= 0) return hipSuccess;
     switch (prec) {
     case PREC_F64: return launch_p<PREC_F64>(order, lim, C, wpb, nblocks, s, a);

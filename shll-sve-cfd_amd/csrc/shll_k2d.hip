@@ -34,9 +34,10 @@ __global__ __launch_bounds__(64 * WPB) void k_step2d(StepArgs a) {
     constexpr int NS = ORDER + 1;      // rows of state / y fluxes kept: c .. c+ORDER
     constexpr bool CMP = PREC == PREC_C32 || PREC == PREC_C32K;  // (hi, lo) state
 
-    const int wave = (int)blockIdx.x * WPB + uniform((int)threadIdx.x >> 6);
     const LaunchView lv = launch_view(a);
-    if (wave >= lv.n_items) return;
+    const int blk = xcd_work((int)blockIdx.x, (lv.n_items + WPB - 1) / WPB);
+    const int wave = blk * WPB + uniform((int)threadIdx.x >> 6);
+    if (blk < 0 || wave >= lv.n_items) return;
     const int lane = (int)threadIdx.x & 63;
     const Item2 it = ((const Item2*)a.items)[wave];
     const Inst in = a.inst[it.inst];
@@ -461,9 +462,10 @@ __device__ __forceinline__ void pipeline2(const StepArgs& a, const Item2& it, co
 
 template <int PREC, int K, int WPB>
 __global__ __launch_bounds__(64 * WPB) void k_step2d_tb(StepArgs a) {
-    const int wave = (int)blockIdx.x * WPB + uniform((int)threadIdx.x >> 6);
     const LaunchView lv = launch_view(a);
-    if (wave >= lv.n_items) return;
+    const int blk = xcd_work((int)blockIdx.x, (lv.n_items + WPB - 1) / WPB);
+    const int wave = blk * WPB + uniform((int)threadIdx.x >> 6);
+    if (blk < 0 || wave >= lv.n_items) return;
     const int lane = (int)threadIdx.x & 63;
     const Item2 it = ((const Item2*)a.items)[wave];
     const Inst in = a.inst[it.inst];
@@ -752,9 +754,10 @@ __device__ __forceinline__ void pipeline2o2(const StepArgs& a, const Item2& it, 
 
 template <int PREC, int K, int WPB, int LIM>
 __global__ __launch_bounds__(64 * WPB) void k_step2d_tb2(StepArgs a) {
-    const int wave = (int)blockIdx.x * WPB + uniform((int)threadIdx.x >> 6);
     const LaunchView lv = launch_view(a);
-    if (wave >= lv.n_items) return;
+    const int blk = xcd_work((int)blockIdx.x, (lv.n_items + WPB - 1) / WPB);
+    const int wave = blk * WPB + uniform((int)threadIdx.x >> 6);
+    if (blk < 0 || wave >= lv.n_items) return;
     const int lane = (int)threadIdx.x & 63;
     const Item2 it = ((const Item2*)a.items)[wave];
     const Inst in = a.inst[it.inst];
@@ -828,7 +831,7 @@ bool step2d_supported(int prec, int order, int k) {
 
 hipError_t launch_step2d(int prec, int order, int lim, int k, int wpb, int n_items, hipStream_t s,
                          const StepArgs& a) {
-    const int nblocks = (n_items + wpb - 1) / wpb;
+    const int nblocks = xcd_grid((n_items + wpb - 1) / wpb);
     if (nblocks <= 0) return hipSuccess;
     if (k != 1 && order == 2) {
         if (prec == PREC_C32) return launch2_tb2<PREC_C32>(k, lim, wpb, nblocks, s, a);

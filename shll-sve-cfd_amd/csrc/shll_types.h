@@ -93,4 +93,19 @@ __host__ __device__ inline LaunchView launch_view(const StepArgs& a) {
     return v;
 }
 
+// XCD-aware work order.  The GPU hands workgroup b of a launch to XCD b % kXcds, and each
+// XCD has its own L2.  With a grid of a multiple of kXcds blocks (the launch dispatchers
+// round it up) and q = ceil(nb / kXcds), block b takes work w = (b % kXcds) * q + b / kXcds,
+// so XCD x works on the contiguous run [x*q, (x+1)*q): neighbouring tiles, whose halos
+// overlap, share an L2.  Every w < nb is taken by exactly one block b < kXcds * q.
+// nb: blocks with work; -1: this block has none.
+constexpr int kXcds = 8;
+__host__ __device__ inline int xcd_work(int b, int nb) {
+    const int q = (nb + kXcds - 1) / kXcds;
+    const int r = b / kXcds;
+    const int w = (b % kXcds) * q + r;
+    return (r < q && w < nb) ? w : -1;
+}
+__host__ __device__ inline int xcd_grid(int nb) { return (nb + kXcds - 1) / kXcds * kXcds; }
+
 }  // namespace shll
