@@ -153,6 +153,13 @@ shll_status shll_stage_fluxes(shll_ctx *ctx, double *a, double *fp, double *fm, 
 /* FP64 sums of the conserved variables times the cell volume (mass, momentum(s),
  * energy); out has nvar entries.  No reference counterpart (diagnostic). */
 shll_status shll_conserved_sums(shll_ctx *ctx, double *out);
+/* shll_conserved_sums without allocating anything.  No counterpart in the reference: it is
+ * for callers that cannot allocate device or host memory (serving loops, code that forbids
+ * hipMalloc).  Same kernel, grid and summation order as shll_conserved_sums, so the same bits;
+ * the block partials go to a static device scratch of the library (256 blocks x 6 doubles =
+ * 12 KiB, held by every GPU the library runs on until the process ends) and to the stack.
+ * Calls that overlap on one GPU share that scratch and race: serialise them. */
+shll_status shll_conserved_sums_noalloc(shll_ctx *ctx, double *out);
 /* Scan the state for non-finite or non-positive density/energy: SHLL_OK or
  * SHLL_ENONFINITE; *bad_cells (may be NULL) receives the count. */
 shll_status shll_check_state(shll_ctx *ctx, int64_t *bad_cells);

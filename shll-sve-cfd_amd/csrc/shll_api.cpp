@@ -37,6 +37,7 @@ hipError_t launch_sums(int prec, int dim, int nblocks, int n_inst, hipStream_t s
 hipError_t launch_ctl(hipStream_t st, int64_t* ctl, int64_t v0, int64_t v1, int64_t v2, int add);
 hipError_t launch_compare(int nblocks, hipStream_t st, const double* a, int64_t plane_a, const double* b,
                           int64_t plane_b, int64_t ncell, int dim, double* partial);
+hipError_t noalloc_scratch(int device, double** out, size_t* capacity);
 }  // namespace shll
 
 using namespace shll;
@@ -983,6 +984,34 @@ extern "C" shll_status shll_conserved_sums(shll_ctx* ctx, double* out) {
     if (st != SHLL_OK) return st;
     const double vol = cell_volume(ctx->pb);
     for (int v = 0; v < ctx->E->nv; ++v) out[v] = s[v] * vol;
+    return SHLL_OK;
+}
+
+// shll_conserved_sums without any allocation: the same kernel, grid and summation order,
+// with the block partials in the library's static device scratch and on this stack
+extern "C" shll_status shll_conserved_sums_noalloc(shll_ctx* ctx, double* out) {
+    if (!ctx || !out) return fail(SHLL_EINVAL, "NULL argument");
+    Engine* E = ctx->E;
+    HIPCHECK(hipSetDevice(E->device));
+    double* scratch = nullptr;
+    size_t cap = 0;
+    HIPCHECK(noalloc_scratch(E->device, &scratch, &cap));
+    constexpr int kMaxBlocks = 256;
+    const int nb = std::min(elem_blocks(E->inst_cells[0]), kMaxBlocks);
+    if ((size_t)nb * 6 > cap)
+        return fail(SHLL_EINVAL, "%d partial blocks exceed the %zu-double scratch of the allocation-free sums", nb,
+                    cap);
+    HIPCHECK(launch_sums(E->prec, E->dim, nb, 1, E->stream, E->d_inst, E->buf[E->cur][0], E->buf[E->cur][1],
+                         E->plane, scratch));
+    double part[kMaxBlocks * 6];
+    HIPCHECK(hipMemcpyAsync(part, scratch, (size_t)nb * 6 * sizeof(double), hipMemcpyDeviceToHost, E->stream));
+    HIPCHECK(hipStreamSynchronize(E->stream));
+    const double vol = cell_volume(ctx->pb);
+    for (int v = 0; v < E->nv; ++v) {
+        double t = 0;
+        for (int b = 0; b < nb; ++b) t += part[(size_t)b * 6 + v];
+        out[v] = t * vol;
+    }
     return SHLL_OK;
 }
 

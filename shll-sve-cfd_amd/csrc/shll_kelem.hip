@@ -12,6 +12,8 @@
 // Reductions: per-lane sums in a fixed cell order, a wave64 xor-butterfly
 // (__shfl_xor), then the waves of a block in index order; blocks write partials
 // that the host adds in index order, so results are deterministic.
+#include <mutex>
+
 #include "shll_flux.h"
 
 namespace shll {
@@ -360,6 +362,37 @@ hipError_t launch_sums(int prec, int dim, int nblocks, int n_inst, hipStream_t s
     if (prec == PREC_F64 || prec == PREC_F64F) return sums_p<PREC_F64>(dim, g, st, insts, s0, s1, plane, partial);
     if (prec == PREC_REF32) return sums_p<PREC_REF32>(dim, g, st, insts, s0, s1, plane, partial);
     return sums_p<PREC_C32>(dim, g, st, insts, s0, s1, plane, partial);
+}
+
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free entry point (shll_conserved_sums_noalloc): the block
+// partials of k_sums for one instance, at most 256 blocks of 6 doubles (the sums kernels
+// never use more blocks: their grid strides over the cells).  Every GPU holds its own copy
+// of the array at its own address; host code must use the address hipGetSymbolAddress
+// returns on that device, never the array's name.  k_sums writes every slot it uses, so
+// nothing needs clearing between calls; calls that overlap on one GPU share it and race.
+// ---------------------------------------------------------------------------
+constexpr int kNoallocScratchDoubles = 256 * 6;
+static __device__ __align__(256) double scratch[kNoallocScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kMaxDevices] = {};
+
+// device address of `scratch` on `device` (which must be the current device), looked up
+// once per device; capacity in doubles
+hipError_t noalloc_scratch(int device, double** out, size_t* capacity) {
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[device] = (double*)p;
+    }
+    *out = g_scratch_ptr[device];
+    *capacity = kNoallocScratchDoubles;
+    return hipSuccess;
 }
 
 hipError_t launch_compare(int nblocks, hipStream_t st, const double* a, int64_t plane_a, const double* b,

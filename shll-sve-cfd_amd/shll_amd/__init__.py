@@ -68,7 +68,7 @@ EXPORTS = [
     "shll_init", "shll_step", "shll_run_to_time", "shll_steps_to_time", "shll_get_primitives",
     "shll_get_conserved", "shll_state_bytes", "shll_get_state_raw", "shll_set_state_raw", "shll_write_results",
     "shll_write_results_full", "shll_stage_fluxes",
-    "shll_conserved_sums", "shll_check_state", "shll_compare", "shll_compare_host", "shll_free",
+    "shll_conserved_sums", "shll_conserved_sums_noalloc", "shll_check_state", "shll_compare", "shll_compare_host", "shll_free",
     "shll_set_launch", "shll_get_launch", "shll_autotune", "shll_set_timing", "shll_last_timing",
     "shll_batch_init", "shll_batch_advance", "shll_batch_run", "shll_batch_get_primitives",
     "shll_batch_get_conserved", "shll_batch_conserved_sums", "shll_batch_set_launch",
@@ -106,6 +106,7 @@ def lib():
         "shll_write_results": (c, [vp, ctypes.c_char_p]),
         "shll_write_results_full": (c, [vp, ctypes.c_char_p]),
         "shll_conserved_sums": (c, [vp, dp]),
+        "shll_conserved_sums_noalloc": (c, [vp, dp]),
         "shll_stage_fluxes": (c, [vp, dp, dp, dp, dp, dp]),
         "shll_check_state": (c, [vp, ctypes.POINTER(i64)]),
         "shll_compare": (c, [vp, vp, ctypes.POINTER(Errors)]),
