@@ -28,7 +28,8 @@ roofline: the timed kernel reads and writes HBM once per k steps, so its bound i
 instruction issue.  frac is the measured HBM traffic (rocprofv3 FETCH_SIZE + WRITE_SIZE,
 calibrated by a copy of known size; profiles/) over the event time and the 8 TB/s peak;
 valu.frac is the kernel's VALU instructions per launch (rocprofv3 SQ_INSTS_VALU) over the
-event time and the measured FP64 issue ceiling (profiles/valu_issue_ceiling.md);
+event time and the issue ceiling of its own instruction mix (per-class rates measured in
+profiles/valu_issue_ceiling.md);
 algorithmic_GBps is 48 B per cell-update (read and write the state once) over the event
 time, which temporal blocking lets exceed the peak.
 
@@ -106,20 +107,13 @@ def profile_figure(summary, key, shape):
     return float(m.group(1))
 
 
-def valu_ceiling():
-    """Measured FP32 and FP64 VALU issue ceilings (wavefront instructions per SIMD per ns)
-    and the SIMD count, from profiles/valu_issue_ceiling.md."""
+def simd_count():
+    """SIMDs of the GPU the issue ceilings were measured on (profiles/valu_issue_ceiling.md)."""
     path = os.path.join(PROFILES, "valu_issue_ceiling.md")
     if not os.path.exists(path):
         return None
-    t = open(path).read()
-    out = {}
-    for key in ("fp32_per_simd_per_ns", "fp64_per_simd_per_ns", "simds", "clock_GHz"):
-        m = re.search(r"%s:\s*([0-9.eE+-]+)" % key, t)
-        if not m:
-            return None
-        out[key] = float(m.group(1))
-    return out
+    m = re.search(r"simds:\s*([0-9]+)", open(path).read())
+    return int(m.group(1)) if m else None
 
 
 def cpu_model():
@@ -530,25 +524,26 @@ def cpu_baseline_2d(pbs, prims, nthreads=1, steps=40):
                           len(idx), steps, total, secs, how, cpu_model())}
 
 
-def roofline(kind, upd, launches, ev_ms, avg_ms, B, shape, traffic_summary, valu_summary, kernel, prec_dtype):
+def roofline(upd, launches, ev_ms, avg_ms, B, shape, traffic_summary, valu_summary, kernel):
     """Roofline object of the timed step kernel: measured HBM traffic against the 8 TB/s
     peak (frac), the VALU issue fraction, and the algorithmic bytes rate."""
     t = ev_ms * 1e-3
     per_launch = upd / max(1, launches)
     tpu = profile_figure(traffic_summary, "hbm_bytes_per_cell_update", shape)
     vpu = profile_figure(valu_summary, "valu_insts_per_cell_update", shape)
-    ceil = valu_ceiling()
+    ceil = profile_figure(valu_summary, "valu_mix_ceiling_per_simd_per_ns", shape)
+    simds = simd_count()
     traffic = tpu * per_launch if tpu is not None else None
     achieved = tpu * upd / t / 1e9 if (tpu is not None and t > 0) else None
     valu = None
-    if vpu is not None and ceil is not None and t > 0:
-        per_ns = ceil["fp64_per_simd_per_ns"] if prec_dtype == "f64" else ceil["fp32_per_simd_per_ns"]
+    if vpu is not None and ceil is not None and simds and t > 0:
         rate = vpu * upd / t  # wavefront VALU instructions per second
-        valu = {"achieved_wave_insts_per_s": rate, "peak_wave_insts_per_s": per_ns * 1e9 * ceil["simds"],
-                "frac": rate / (per_ns * 1e9 * ceil["simds"]), "insts_per_cell_update": vpu,
-                "ceiling": "%s fma issue, measured (profiles/valu_issue_ceiling.md, %.4g per SIMD per ns, "
-                           "%d SIMDs, max clock %.2f GHz)" % ("FP64" if prec_dtype == "f64" else "FP32", per_ns,
-                                                             ceil["simds"], ceil["clock_GHz"])}
+        peak = ceil * 1e9 * simds
+        valu = {"achieved": rate, "peak": peak, "unit": "wavefront VALU instructions/s", "frac": rate / peak,
+                "insts_per_cell_update": vpu,
+                "ceiling": "the kernel's interior-loop instruction mix weighted by the measured per-class issue "
+                           "rates (%.4f per SIMD per ns, %d SIMDs; profiles/%s, profiles/valu_issue_ceiling.md)"
+                           % (ceil, simds, valu_summary)}
     k = shape[1]
     return {
         "bound": "valu" if k > 1 else "hbm",
@@ -612,9 +607,9 @@ def main_domains(S, args):
                    "cells": int(sum(p.nx * p.ny for p in pbs)), "cell_updates_timed": int(upd),
                    "launches": launches, "parallelism": "1 GPU",
                    "launch_shape": {"steps_per_launch": lau[1], "waves_per_block": lau[2], "graph_steps": lau[3]}},
-        "roofline": roofline("2d", upd, launches, ev_ms, avg_ms, B, lau,
+        "roofline": roofline(upd, launches, ev_ms, avg_ms, B, lau,
                              "k_step2d_tb_%s_domains64_hbm_traffic.md" % form,
-                             "k_step2d_tb_%s_domains64_sq_counters.md" % form, kern, FORMS[form][3]),
+                             "k_step2d_tb_%s_domains64_sq_counters.md" % form, kern),
         "accuracy": acc,
     }
     if not args.no_cpu:
@@ -728,9 +723,9 @@ def main():
             "launch_shape": {"cells_per_lane": lau[0], "steps_per_launch": lau[1], "waves_per_block": lau[2],
                              "graph_steps": lau[3]},
         },
-        "roofline": roofline("1d", upd, launches, ev_ms, avg_ms, B, lau,
+        "roofline": roofline(upd, launches, ev_ms, avg_ms, B, lau,
                              "k_step1d_%s_o2_tubes4096_hbm_traffic.md" % args.form,
-                             "k_step1d_%s_o2_tubes4096_sq_counters.md" % args.form, kern, FORMS[args.form][3]),
+                             "k_step1d_%s_o2_tubes4096_sq_counters.md" % args.form, kern),
         "accuracy": acc,
     }
     if not args.no_cpu:

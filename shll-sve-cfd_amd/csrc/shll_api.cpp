@@ -343,6 +343,10 @@ static int64_t batch_total_steps(const Engine* E) {
 
 static shll_status engine_build_items(Engine* E) {
     if (E->items_valid) return SHLL_OK;
+    // the graphs hold pointers to the tables rebuilt below: let any launch of them finish and
+    // drop them before a table is freed or reallocated
+    HIPCHECK(hipStreamSynchronize(E->stream));
+    engine_drop_graphs(E);
     std::vector<Item1> it1;
     std::vector<Item2> it2;
     E->inst_first_item.assign(E->order_idx.size() + 1, 0);
@@ -373,8 +377,6 @@ static shll_status engine_build_items(Engine* E) {
     HIPCHECK(hipMemcpyAsync(E->d_items, E->dim == 1 ? (const void*)it1.data() : (const void*)it2.data(), bytes,
                             hipMemcpyHostToDevice, E->stream));
     HIPCHECK(hipStreamSynchronize(E->stream));
-    // the graphs hold the old pointers: drop them before any buffer they read is freed
-    engine_drop_graphs(E);
     if (E->batch) {
         // item counts of a grid of k-step launches (at most one launch per k steps of the
         // longest instance, whatever step it starts at), read by the constant graph nodes

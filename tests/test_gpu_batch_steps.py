@@ -1,9 +1,9 @@
 """GPU: a batch advances by exactly the steps asked for.  shll_batch_advance(n) with n not
 a multiple of the launch's steps ends in a shorter launch; graph chunks that follow start
 at any step.  After every call each instance equals its own oracle run at exactly
-min(steps taken, its step count); the C32K batch is also checked at the launch shape the
-benchmark times (8 cells per lane, 16 steps per launch, one wavefront per block, 128-step
-graphs) with multi-segment tubes and full graph chunks."""
+min(steps taken, its step count); the F64F and C32K batches are also checked at the launch
+shapes the benchmark times (8 cells per lane, 8 or 16 steps per launch, one wavefront per block,
+128-step graphs) with multi-segment tubes and full graph chunks."""
 import numpy as np
 import pytest
 
@@ -59,15 +59,17 @@ def test_advance_takes_exactly_the_steps_asked_for(orc, prec, dim):
     b.close()
 
 
-def test_headline_launch_shape_batch(orc):
-    """C32K tubes of 4096-9000 cells (several 512-cell segments each, 32-cell halos) at the
-    benchmark's launch shape: a 20-step advance (one 16-step and one 4-step launch), then
-    graph chunks of 8 launches from step 20 on, instances finishing inside chunks."""
-    pbs, prims, steps = S.tube_batch(97531, 6, nmin=4096, nmax=9000, precision=S.C32K)
+@pytest.mark.parametrize("prec,launch", [(S.F64F, (8, 8, 1, 128)), (S.C32K, (8, 16, 1, 128))])
+def test_headline_launch_shape_batch(orc, prec, launch):
+    """Tubes of 4096-9000 cells (several 512-cell segments each) at the benchmark's launch
+    shapes (F64F, the timed form: 8 steps per launch, 16-cell halos; C32K: 16 steps, 32-cell
+    halos): a 20-step advance (a shorter last launch), then 128-step graph chunks from step 20
+    on, instances finishing inside chunks."""
+    pbs, prims, steps = S.tube_batch(97531, 6, nmin=4096, nmax=9000, precision=prec)
     steps = [310, 20, 251, 137, 300, 129]
     b = S.Batch(pbs, prims, steps)
-    b.set_launch(8, 16, 1, 128)
-    banner("headline shape batch", [p.nx for p in pbs])
+    b.set_launch(*launch)
+    banner("headline shape batch", prec, launch, [p.nx for p in pbs])
     assert b.advance(20) == 290
     check_all(orc, b, pbs, prims, steps, 20, "after 20 steps")
     b.run()
