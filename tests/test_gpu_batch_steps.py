@@ -11,7 +11,7 @@ import shll_amd as S
 
 pytestmark = pytest.mark.gpu
 
-MODES = {S.F64: 0, S.REF32: 1, S.C32: 2, S.C32K: 3}
+MODES = {S.F64: 0, S.REF32: 1, S.C32: 2, S.C32K: 3, S.F64F: 4}
 
 
 def banner(*a):
