@@ -55,9 +55,9 @@ def main():
         with S.Solver(pb, S.ic_tube(pb.nx)) as s:
             s.autotune(trial)
             print("single dim1 order%d prec%d N=%d:" % (pb.order, pb.precision, pb.nx), s.get_launch(), flush=True)
-    # 2D first order: the 64-domain batch (both compensated forms) and one 4096^2 domain
+    # 2D first order: the 64-domain batch (both compensated forms and FP64) and one 4096^2 domain
     # (every precision); a 256-step trial covers 64 launches of 4 steps
-    for prec in (S.C32K, S.C32):
+    for prec in (S.C32K, S.C32, S.F64):
         if not wanted(prec):
             continue
         pbs, prims = S.domain_batch(4242, 64, precision=prec)
