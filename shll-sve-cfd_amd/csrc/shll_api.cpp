@@ -1235,8 +1235,14 @@ static shll_status engine_autotune(Engine* E, int32_t trial_steps, int64_t shape
     std::vector<Launch> cands;
     if (E->dim == 1) {
         for (int C : {1, 2, 4, 8})
-            for (int k : {1, 2, 4, 8, 16, 32, 64})
-                for (int w : {1, 4}) {
+            for (int w : {1, 4}) {
+                std::vector<int> ks = {1, 2, 4, 8, 16, 32, 64};
+                // a single instance that fits one segment or cooperative block has no halo, so
+                // any number of steps fits one launch: try long launches too
+                const int XW = step1d_items_per_block(C, w) == 1 && w > 1 ? w : 1;
+                if (!E->batch && E->inst_cells[0] + 2 <= 64LL * C * XW)
+                    for (int k : {256, 1024, 4096}) ks.push_back(k);
+                for (int k : ks) {
                     // batches: fewer than 4 steps per launch only multiplies the launches
                     if (E->batch && k < 4) continue;
                     Launch L = start;
@@ -1244,6 +1250,7 @@ static shll_status engine_autotune(Engine* E, int32_t trial_steps, int64_t shape
                     L.graph = std::max(128, 2 * k);
                     cands.push_back(L);
                 }
+            }
     } else {
         for (int k : {1, 2, 3, 4, 8})
             for (int w : {1, 4}) {

@@ -94,6 +94,23 @@ def test_1d_cooperative_blocks_wall_at_a_wavefront_edge(orc, launch, sizes, bc):
         assert_same(pb, gpu_run(pb, ic, 40, launch), oracle_run(orc, pb, ic, 40), "coop %s n=%d" % (launch, n))
 
 
+@pytest.mark.parametrize("prec", [S.F64, S.REF32, S.C32K, S.F64F])
+def test_1d_long_launches_without_halo(orc, prec):
+    """A tube that fits one segment (or one cooperative block) has no halo, so any number of
+    steps fits one launch: Sod at 1024 cells to t = 0.2 in one 1639-step launch (8 cells per
+    lane, a block of 4 wavefronts), and a 500-cell tube in launches of 1024 steps (one
+    wavefront), both walls; bitwise."""
+    pb = S.problem(1, 1, 1024, precision=prec)
+    ic = S.ic_tube(1024)
+    banner("long launch 1024", prec)
+    assert_same(pb, gpu_run(pb, ic, 1639, (8, 4096, 4, 8192)), oracle_run(orc, pb, ic, 1639), "1639-step launch")
+    for bc in (0, 1):
+        pb = S.problem(1, 2, 500, precision=prec, bc=bc)
+        ic = icgen.random_1d(500, 82, block=20)
+        banner("long launch 500", prec, bc)
+        assert_same(pb, gpu_run(pb, ic, 2500, (8, 1024, 1, 0)), oracle_run(orc, pb, ic, 2500), "1024-step launches")
+
+
 def test_1d_impossible_launch_shape_is_rejected():
     pb = S.problem(1, 2, 3001)
     with S.Solver(pb, S.ic_tube(3001)) as s:
