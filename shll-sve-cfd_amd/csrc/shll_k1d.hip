@@ -163,6 +163,53 @@ __device__ __forceinline__ void make_window_or(const RT (&q)[C], RT (&W)[C + 4],
     W[C + 3] = from_right_or(q[1], e1);
 }
 
+// Second order with independent wavefronts (XW = 1, C >= 2): windows of one neighbour cell
+// per side, W[p+1] = value at lane position p in [-1, C], and each limited slope computed
+// once.  A lane limits its own C cells; the F+ slope of cell -1 and the F- slope of cell C
+// are the neighbour lanes' slopes of those same cells, from the same window values (so the
+// same bits), moved over by a lane shift.  sp(f) / sm(f): the F+ slope of cell f-1 and the
+// F- slope of cell f, the two slopes face f uses; the interior ones are computed where they
+// are used, so only the two edge slopes stay live across the faces.  Edge lanes of the
+// segment get don't-care values, as the wider windows give them.
+template <class A, int WM, int LIM, class RT, int C, bool ZERO>
+struct SharedSlopes {
+    static_assert(C >= 2, "a lane's own slopes need two cells");
+    RT FP[C + 2], FM[C + 2];
+    RT sP_in, sP_last, sM_first, sM_in;
+    int gl, n;
+    RT th;
+    __device__ __forceinline__ SharedSlopes(const RT (&fp)[C], const RT (&fm)[C], int gl_, int n_, RT th_)
+        : gl(gl_), n(n_), th(th_) {
+#pragma unroll
+        for (int m = 0; m < C; ++m) {
+            FP[m + 1] = fp[m];
+            FM[m + 1] = fm[m];
+        }
+        if constexpr (ZERO) {
+            FP[0] = from_left_z(fp[C - 1]);
+            FP[C + 1] = from_right_z(fp[0]);
+            FM[0] = from_left_z(fm[C - 1]);
+            FM[C + 1] = from_right_z(fm[0]);
+            asm volatile("" : "+v"(FP[0]), "+v"(FP[C + 1]), "+v"(FM[0]), "+v"(FM[C + 1]));
+        } else {
+            FP[0] = from_left(fp[C - 1]);
+            FP[C + 1] = from_right(fp[0]);
+            FM[0] = from_left(fm[C - 1]);
+            FM[C + 1] = from_right(fm[0]);
+        }
+        sP_last = slope_at<A, WM, LIM>(FP, C - 2, gl + C - 1, n, th);  // F+ slope of cell C-1
+        sM_first = slope_at<A, WM, LIM>(FM, -1, gl, n, th);            // F- slope of cell 0
+        sP_in = from_left(sP_last);
+        sM_in = from_right(sM_first);
+    }
+    __device__ __forceinline__ RT sp(int f) const {
+        return f == 0 ? sP_in : f == C ? sP_last : slope_at<A, WM, LIM>(FP, f - 2, gl + f - 1, n, th);
+    }
+    __device__ __forceinline__ RT sm(int f) const {
+        return f == C ? sM_in : f == 0 ? sM_first : slope_at<A, WM, LIM>(FM, f - 1, gl + f, n, th);
+    }
+};
+
 // Walls as ghost cells: before each step the state of cell -1 becomes that of cell 0
 // and the state of cell n that of cell n-1 -- copied for outflow walls, mirrored
 // (momentum negated) for reflective ones.  Every operation of the split flux is odd
@@ -222,6 +269,21 @@ __device__ __forceinline__ void step_ref(typename Real<PREC>::T (&u)[3][C], cons
             fm[v][m] = q[v];
         }
     }
+    if constexpr (ORDER == 2 && XW == 1 && C >= 2) {
+#pragma unroll
+        for (int v = 0; v < 3; ++v) {
+            const SharedSlopes<A, WM, LIM, RT, C, false> w(fp[v], fm[v], gl, n, th);
+#pragma unroll
+            for (int m = 0; m < C; ++m) {
+                // F+ of the left neighbour, F- of the right one; own slopes, the left
+                // neighbour's F+ slope and the right neighbour's F- slope
+                RT uu = u[v][m] - lam * (fp[v][m] - fm[v][m] + w.FM[m + 2] - w.FP[m]);
+                uu = (RT)((double)uu - 0.5 * (double)lam * (double)(w.sp(m + 1) + w.sm(m) - w.sm(m + 1) - w.sp(m)));
+                u[v][m] = uu;
+            }
+        }
+        return;
+    }
     if constexpr (XW > 1) coop_put<RT, C, XW>(x, parity, fp, fm);
 #pragma unroll
     for (int v = 0; v < 3; ++v) {
@@ -259,6 +321,23 @@ __device__ __forceinline__ void step_ref(typename Real<PREC>::T (&u)[3][C], cons
 // ---------------------------------------------------------------------------
 // KAHAN: the face terms t[m] - t[m+1] enter each cell by one Kahan step (lo holds
 // the part of the value not yet in hi) instead of the exact TwoSum chain
+template <bool KAHAN>
+__device__ __forceinline__ void c32_update(float& hi, float& lo, float tl, float tr) {
+    if constexpr (KAHAN) {
+        const float y = (tl - tr) + lo;
+        const float s = hi + y;
+        lo = y - (s - hi);
+        hi = s;
+    } else {
+        float a = hi, b = lo, e;
+        two_sum(a, tl, a, e);
+        b = b + e;
+        two_sum(a, -tr, a, e);
+        b = b + e;
+        fast_two_sum(a, b, hi, lo);
+    }
+}
+
 template <int ORDER, int C, int WM, bool KAHAN, int LIM, int XW>
 __device__ __forceinline__ void step_c32(float (&hi)[3][C], float (&lo)[3][C], const Inst& in, int gl,
                                          const Coop<float, XW>& x, int parity) {
@@ -276,6 +355,21 @@ __device__ __forceinline__ void step_c32(float (&hi)[3][C], float (&lo)[3][C], c
             fp[v][m] = p[v];
             fm[v][m] = q[v];
         }
+    }
+    // shared slopes up to 4 cells per lane: at 8 the FP32 kernels ran 4% slower with them
+    // (their schedule needed 168 registers instead of 164) than they gain in instructions
+    if constexpr (ORDER == 2 && XW == 1 && C >= 2 && C <= 4) {
+#pragma unroll
+        for (int v = 0; v < 3; ++v) {
+            const SharedSlopes<A, WM, LIM, float, C, !KAHAN> w(fp[v], fm[v], gl, n, th);
+            float t[C + 1];
+#pragma unroll
+            for (int f = 0; f <= C; ++f)  // face f lies between cells gl+f-1 and gl+f
+                t[f] = __builtin_fmaf(hlam, w.sp(f) - w.sm(f), lam * (w.FP[f] + w.FM[f + 1]));
+#pragma unroll
+            for (int m = 0; m < C; ++m) c32_update<KAHAN>(hi[v][m], lo[v][m], t[m], t[m + 1]);
+        }
+        return;
     }
     if constexpr (XW > 1) coop_put<float, C, XW>(x, parity, fp, fm);
 #pragma unroll
@@ -302,21 +396,7 @@ __device__ __forceinline__ void step_c32(float (&hi)[3][C], float (&lo)[3][C], c
             }
         }
 #pragma unroll
-        for (int m = 0; m < C; ++m) {
-            if constexpr (KAHAN) {
-                const float y = (t[m] - t[m + 1]) + lo[v][m];
-                const float s = hi[v][m] + y;
-                lo[v][m] = y - (s - hi[v][m]);
-                hi[v][m] = s;
-            } else {
-                float a = hi[v][m], b = lo[v][m], e;
-                two_sum(a, t[m], a, e);
-                b = b + e;
-                two_sum(a, -t[m + 1], a, e);
-                b = b + e;
-                fast_two_sum(a, b, hi[v][m], lo[v][m]);
-            }
-        }
+        for (int m = 0; m < C; ++m) c32_update<KAHAN>(hi[v][m], lo[v][m], t[m], t[m + 1]);
     }
 }
 
@@ -341,6 +421,19 @@ __device__ __forceinline__ void step_f64f(double (&u)[3][C], const Inst& in, int
             fp[v][m] = p[v];
             fm[v][m] = q[v];
         }
+    }
+    if constexpr (ORDER == 2 && XW == 1 && C >= 2) {
+#pragma unroll
+        for (int v = 0; v < 3; ++v) {
+            const SharedSlopes<A, WM, LIM, double, C, false> w(fp[v], fm[v], gl, n, th);
+            double t[C + 1];
+#pragma unroll
+            for (int f = 0; f <= C; ++f)  // face f lies between cells gl+f-1 and gl+f
+                t[f] = __builtin_fma(hlam, w.sp(f) - w.sm(f), lam * (w.FP[f] + w.FM[f + 1]));  // lam*(G + d/2)
+#pragma unroll
+            for (int m = 0; m < C; ++m) u[v][m] = u[v][m] + (t[m] - t[m + 1]);
+        }
+        return;
     }
     if constexpr (XW > 1) coop_put<double, C, XW>(x, parity, fp, fm);
 #pragma unroll
@@ -434,8 +527,18 @@ __device__ __forceinline__ void advance_ref(typename Real<PREC>::T (&u)[3][C], c
 // ---------------------------------------------------------------------------
 // the launch: one wavefront per Item1 (LIM: the second-order limiter)
 // ---------------------------------------------------------------------------
+// Wavefronts per SIMD the register allocator must leave room for (1: no bound).  Left to
+// itself it spends registers on scheduling freedom in the second-order F64F kernel with
+// independent wavefronts at 8 cells per lane and drops it to one wavefront per SIMD; bound
+// to 2, it spills a few values outside the step loop.
+template <int PREC, int ORDER, int C, int WPB>
+constexpr int step1d_min_waves() {
+    return PREC == PREC_F64F && ORDER == 2 && C == 8 && WPB == 1 ? 2 : 1;
+}
+
 template <int PREC, int ORDER, int C, int WPB, int LIM>
-__global__ __launch_bounds__(64 * WPB) void k_step1d(StepArgs a) {
+__global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(step1d_min_waves<PREC, ORDER, C, WPB>())))
+void k_step1d(StepArgs a) {
     using RT = typename Real<PREC>::T;
     // blocks of several wavefronts with at least 2 cells per lane are cooperative: one item
     // per block, one segment of 64*C cells per wavefront
