@@ -190,7 +190,8 @@ struct SharedSlopes {
             FP[C + 1] = from_right_z(fp[0]);
             FM[0] = from_left_z(fm[C - 1]);
             FM[C + 1] = from_right_z(fm[0]);
-            asm volatile("" : "+v"(FP[0]), "+v"(FP[C + 1]), "+v"(FM[0]), "+v"(FM[C + 1]));
+            if constexpr (sizeof(RT) == 4)  // FP32: keep the shifts as moves (see make_window)
+                asm volatile("" : "+v"(FP[0]), "+v"(FP[C + 1]), "+v"(FM[0]), "+v"(FM[C + 1]));
         } else {
             FP[0] = from_left(fp[C - 1]);
             FP[C + 1] = from_right(fp[0]);
@@ -199,8 +200,13 @@ struct SharedSlopes {
         }
         sP_last = slope_at<A, WM, LIM>(FP, C - 2, gl + C - 1, n, th);  // F+ slope of cell C-1
         sM_first = slope_at<A, WM, LIM>(FM, -1, gl, n, th);            // F- slope of cell 0
-        sP_in = from_left(sP_last);
-        sM_in = from_right(sM_first);
+        if constexpr (ZERO) {
+            sP_in = from_left_z(sP_last);
+            sM_in = from_right_z(sM_first);
+        } else {
+            sP_in = from_left(sP_last);
+            sM_in = from_right(sM_first);
+        }
     }
     __device__ __forceinline__ RT sp(int f) const {
         return f == 0 ? sP_in : f == C ? sP_last : slope_at<A, WM, LIM>(FP, f - 2, gl + f - 1, n, th);
@@ -272,7 +278,8 @@ __device__ __forceinline__ void step_ref(typename Real<PREC>::T (&u)[3][C], cons
     if constexpr (ORDER == 2 && XW == 1 && C >= 2) {
 #pragma unroll
         for (int v = 0; v < 3; ++v) {
-            const SharedSlopes<A, WM, LIM, RT, C, false> w(fp[v], fm[v], gl, n, th);
+            // FP64: zero-filled edge lanes, no copies of the shifted values' old contents
+            const SharedSlopes<A, WM, LIM, RT, C, sizeof(RT) == 8> w(fp[v], fm[v], gl, n, th);
 #pragma unroll
             for (int m = 0; m < C; ++m) {
                 // F+ of the left neighbour, F- of the right one; own slopes, the left
@@ -425,7 +432,7 @@ __device__ __forceinline__ void step_f64f(double (&u)[3][C], const Inst& in, int
     if constexpr (ORDER == 2 && XW == 1 && C >= 2) {
 #pragma unroll
         for (int v = 0; v < 3; ++v) {
-            const SharedSlopes<A, WM, LIM, double, C, false> w(fp[v], fm[v], gl, n, th);
+            const SharedSlopes<A, WM, LIM, double, C, true> w(fp[v], fm[v], gl, n, th);
             double t[C + 1];
 #pragma unroll
             for (int f = 0; f <= C; ++f)  // face f lies between cells gl+f-1 and gl+f
