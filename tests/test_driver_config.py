@@ -1,0 +1,104 @@
+"""The C host driver's options without a GPU: flags and a JSON problem file resolve to the
+same problem (--print-config exits before any GPU call), every value is validated, and the
+printed configuration reads back as a configuration.  Defaults follow the reference files'
+compile-time constants (base_shll.c:16-26, base_shll_2d.c:21-35, 2nd_order_base_shll.c:
+21-43, base-omp/2nd_order_base_shll.c:43-44)."""
+import json
+import os
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "shll-sve-cfd_amd", "bin", "shll_run")
+
+
+def run(*args, cwd=None):
+    r = subprocess.run([DRIVER, *args], capture_output=True, text=True, timeout=30, cwd=cwd)
+    return r.returncode, r.stdout, r.stderr
+
+
+def resolved(*args, cwd=None):
+    rc, out, err = run(*args, "--print-config", cwd=cwd)
+    assert rc == 0, err
+    return json.loads(out)
+
+
+def test_defaults_are_the_reference_files():
+    c = resolved()
+    assert (c["dim"], c["order"], c["nx"], c["ny"], c["precision"], c["bc"], c["ic"]) == \
+        (1, 1, 256, 1, "ref32", "reflect", "sod")
+    assert c["t_end"] == pytest.approx(0.2, rel=1e-7) and c["steps"] == -1 and c["out"] == "results.dat"
+    c = resolved("--dim", "2")
+    assert (c["ny"], c["bc"], c["ic"]) == (256, "reflect", "implosion") and c["t_end"] == pytest.approx(0.1, rel=1e-7)
+    c = resolved("--dim", "2", "--order", "2")
+    assert (c["bc"], c["ic"], c["limiter"]) == ("outflow", "four-shocks", "minmod")
+    assert c["t_end"] == pytest.approx(0.8, rel=1e-7)
+    c = resolved("--dim", "2", "--order", "2", "--nx", "1024", "--limiter", "mc")
+    assert (c["ic"], c["theta"], c["ny"]) == ("four-contacts", 1.25, 1024) and c["t_end"] == pytest.approx(0.3, rel=1e-7)
+
+
+def test_json_file_equals_flags_and_later_flags_override(tmp_path):
+    cfg = {"dim": 2, "order": 2, "nx": 512, "ny": 384, "precision": "c32k", "bc": "reflect", "ic": "circle",
+           "limiter": "mc", "theta": 1.5, "t_end": 0.05, "steps": 40, "out": "a b.dat", "dump": "s.bin",
+           "full": True, "sums": True}
+    (tmp_path / "c.json").write_text(json.dumps(cfg, indent=2))
+    flags = ["--dim", "2", "--order", "2", "--nx", "512", "--ny", "384", "--precision", "c32k", "--bc", "reflect",
+             "--ic", "circle", "--limiter", "mc", "--theta", "1.5", "--t-end", "0.05", "--steps", "40",
+             "--out", "a b.dat", "--dump", "s.bin", "--full", "--sums"]
+    a = resolved("--config", str(tmp_path / "c.json"))
+    assert a == resolved(*flags)
+    for k in cfg:
+        if k != "t_end":
+            assert a[k] == cfg[k], k
+    b = resolved("--config", str(tmp_path / "c.json"), "--precision", "f64", "--nx", "64")
+    assert (b["precision"], b["nx"], b["ny"]) == ("f64", 64, 384)
+    # flags before the file are overridden by it
+    assert resolved("--precision", "f64", "--config", str(tmp_path / "c.json"))["precision"] == "c32k"
+
+
+def test_printed_configuration_reads_back(tmp_path):
+    for args in ((), ("--dim", "2", "--order", "2", "--limiter", "mc", "--nx", "100", "--precision", "f64"),
+                 ("--precision", "f64f", "--order", "2", "--out", 'q"uote\\d.dat', "--steps", "7")):
+        first = resolved(*args)
+        (tmp_path / "p.json").write_text(json.dumps(first))
+        assert resolved("--config", str(tmp_path / "p.json")) == first, args
+
+
+@pytest.mark.parametrize("args,msg", [
+    (("--precision", "f65"), "bad value 'f65' for 'precision'"),
+    (("--nx", "12x"), "bad value '12x' for 'nx'"),
+    (("--dim", "3"), "bad value '3' for 'dim'"),
+    (("--bc", "periodic"), "bad value 'periodic' for 'bc'"),
+    (("--limiter", "superbee"), "bad value 'superbee' for 'limiter'"),
+    (("--t-end", "-1"), "bad value '-1' for 't-end'"),
+    (("--ic", "implosion"), "initial data 'implosion' is 2D"),
+    (("--dim", "2", "--ic", "sod"), "initial data 'sod' is 1D"),
+    (("--frobnicate", "1"), "unknown option 'frobnicate'"),
+    (("--nx",), "usage:"),
+])
+def test_bad_options_are_rejected(args, msg):
+    rc, out, err = run(*args, "--print-config")
+    assert rc == 2 and msg in err and out == ""
+
+
+@pytest.mark.parametrize("text,msg", [
+    ('{"dim": 2,}', "expected a key"),
+    ('{"dim" 2}', "expected ':'"),
+    ('[1, 2]', "expected a JSON object"),
+    ('{"dim": 2} x', "trailing characters"),
+    ('{"bogus": 1}', "unknown option 'bogus'"),
+    ('{"nx": "many"}', "bad value 'many' for 'nx'"),
+    ('{"full": "yes"}', "bad value 'yes' for 'full'"),
+    ('{"out": "a\\nb"}', "unsupported escape"),
+    ('{"dim": 2', "expected ',' or '}'"),
+])
+def test_bad_files_are_rejected(tmp_path, text, msg):
+    (tmp_path / "c.json").write_text(text)
+    rc, out, err = run("--config", str(tmp_path / "c.json"), "--print-config")
+    assert rc == 2 and msg in err, err
+
+
+def test_missing_file(tmp_path):
+    rc, _, err = run("--config", str(tmp_path / "none.json"))
+    assert rc == 2 and "cannot open" in err

@@ -60,6 +60,21 @@ def test_driver_2d_reference_runs(tmp_path, name, args, steps):
     np.testing.assert_allclose(rows[:, 5], p[3], rtol=6e-7)
 
 
+def test_driver_json_problem_file(tmp_path):
+    """The same 2nd_order_base_shll.c run from a JSON problem file: same stdout, same digests."""
+    import json
+    g = np.load(os.path.join(GOLD, "o2_2d_main.npz"), allow_pickle=False)
+    (tmp_path / "run.json").write_text(json.dumps({"dim": 2, "order": 2, "nx": 256, "dump": "end.bin",
+                                                   "out": "-"}))
+    out = run_driver(tmp_path, "--config", "run.json")
+    assert out == bytes(g["stdout"])
+    n = 256 * 256
+    raw = (tmp_path / "end.bin").read_bytes()
+    u = np.frombuffer(raw[4 * n * 8:], dtype=np.float32).reshape(4, n)
+    assert [hashlib.sha256(np.ascontiguousarray(x).tobytes()).hexdigest() for x in u] == list(g["dig_1639"][4:8])
+    assert not (tmp_path / "results.dat").exists()
+
+
 @pytest.mark.parametrize("prec", [S.F64, S.REF32, S.F64F, S.C32K])
 @pytest.mark.parametrize("order", [1, 2])
 def test_momentum_change_equals_wall_impulse_1d(prec, order):
