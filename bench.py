@@ -336,10 +336,20 @@ def tube_accuracy(S, b, pbs, judge):
 # ---------------------------------------------------------------------------
 # CPU baselines
 # ---------------------------------------------------------------------------
+def bind_threads():
+    """OpenMP threads bound one per core, close together (OMP_PROC_BIND=close,
+    OMP_PLACES=cores), unless the caller set them: for this process (the oracle's OpenMP
+    runs, set before the oracle library starts its runtime) and the reference runs."""
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
+    return "threads bound: OMP_PROC_BIND=%s, OMP_PLACES=%s" % (os.environ["OMP_PROC_BIND"], os.environ["OMP_PLACES"])
+
+
 def cpu_baseline(S, pbs, prims, steps, nthreads=1):
     """Oracle REF32 (the reference's arithmetic and pass structure) on every 256th tube of
     the batch for up to 2000 steps each: serially (the base-c loop), or with one OpenMP
     worksharing loop per pass on nthreads threads (the base-omp pattern)."""
+    bound = bind_threads()
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     idx = list(range(0, len(pbs), 256))
@@ -351,7 +361,7 @@ def cpu_baseline(S, pbs, prims, steps, nthreads=1):
         _, _, el = oracle.run(op, oracle.REF32, prims[k], n, nthreads=nthreads, timed=True)
         total += pb.nx * n
         secs += el
-    how = "1 thread" if nthreads == 1 else "%d OpenMP threads, one worksharing loop per pass" % nthreads
+    how = "1 thread" if nthreads == 1 else "%d OpenMP threads (%s), one worksharing loop per pass" % (nthreads, bound)
     return {"value": total / secs, "unit": "cell-updates/s", "cores": nthreads, "kind": "port",
             "sample": "%d tubes (every 256th of the batch, N=%d..%d), min(steps,2000) steps each, "
                       "%.3g cell-updates in %.1f s; oracle REF32 = reference float arithmetic and "
@@ -386,6 +396,7 @@ def reference_cpu(cores):
          "implosion data), gcc -O3", [(1, 3)]),
     ]
     out = {}
+    bound = bind_threads()
     for exe, what, plan in runs:
         path = os.path.join(ROOT, "oracle", "_ref", exe + ".run")
         if not os.path.exists(path):
@@ -409,7 +420,7 @@ def reference_cpu(cores):
             out["%s_%dt" % (exe, th)] = {
                 "value": cells * st / sec, "unit": "cell-updates/s", "cores": th, "kind": "reference",
                 "sample": "%s: %d cells x %d steps in %.2f s on %d thread%s (step loop only%s); host %s" % (
-                    what, cells, st, sec, th, "" if th == 1 else "s",
+                    what, cells, st, sec, th, "" if th == 1 else "s (%s)" % bound,
                     "" if len(secs) == 1 else ", median of %d runs" % len(secs), cpu_model())}
     return out
 
@@ -543,6 +554,7 @@ DOMAIN_STEPS = 1000
 def cpu_baseline_2d(pbs, prims, nthreads=1, steps=40):
     """Oracle REF32 first-order 2D (the reference's arithmetic and pass structure) on every
     8th domain of the batch for a few steps each."""
+    bound = bind_threads()
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     idx = list(range(0, len(pbs), 8))
@@ -553,7 +565,7 @@ def cpu_baseline_2d(pbs, prims, nthreads=1, steps=40):
         _, _, el = oracle.run(op, oracle.REF32, prims[k], steps, nthreads=nthreads, timed=True)
         total += pb.nx * pb.ny * steps
         secs += el
-    how = "1 thread" if nthreads == 1 else "%d OpenMP threads, one worksharing loop per pass" % nthreads
+    how = "1 thread" if nthreads == 1 else "%d OpenMP threads (%s), one worksharing loop per pass" % (nthreads, bound)
     return {"value": total / secs, "unit": "cell-updates/s", "cores": nthreads, "kind": "port",
             "sample": "%d domains (every 8th of the batch), %d steps each, %.3g cell-updates in %.1f s; oracle "
                       "REF32 = reference float arithmetic and unfused pass structure, gcc -O3, %s; host %s" % (
