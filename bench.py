@@ -63,6 +63,12 @@ import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
+# the CPUs this process may use, before any OpenMP runtime binds its threads (the oracle's
+# OpenMP runs bind the calling thread to the first place, and child processes inherit that)
+try:
+    ALLOWED_CPUS = sorted(os.sched_getaffinity(0))
+except AttributeError:
+    ALLOWED_CPUS = list(range(os.cpu_count() or 1))
 sys.path.insert(0, os.path.join(ROOT, "shll-sve-cfd_amd"))
 # launch shapes chosen by the library's autotuner on MI355X (tune/make_tune_cache.py);
 # --autotune re-tunes live instead
@@ -133,10 +139,7 @@ def cpu_model():
 
 def host_cores():
     """The CPUs this process may use (the box's share of the host)."""
-    try:
-        return len(os.sched_getaffinity(0))
-    except AttributeError:
-        return os.cpu_count() or 1
+    return len(ALLOWED_CPUS)
 
 
 def calibration_copy(nbytes):
@@ -336,13 +339,23 @@ def tube_accuracy(S, b, pbs, judge):
 # ---------------------------------------------------------------------------
 # CPU baselines
 # ---------------------------------------------------------------------------
+def restore_affinity():
+    """The calling thread (or, as a preexec_fn, a child process) back on every CPU this
+    process was given at start."""
+    if hasattr(os, "sched_setaffinity"):
+        os.sched_setaffinity(0, ALLOWED_CPUS)
+
+
 def bind_threads():
-    """OpenMP threads bound one per core, close together (OMP_PROC_BIND=close,
-    OMP_PLACES=cores), unless the caller set them: for this process (the oracle's OpenMP
-    runs, set before the oracle library starts its runtime) and the reference runs."""
+    """OpenMP threads bound close together, one per CPU this process may use
+    (OMP_PROC_BIND=close, OMP_PLACES = an explicit list of those CPUs), unless the caller
+    set them: for this process (the oracle's OpenMP runs, set before the oracle library
+    starts its runtime) and the reference runs.  The list is explicit because OMP_PLACES=cores
+    inside a restricted CPU set put every thread of the LLVM OpenMP runtime on one place."""
+    cpus = ALLOWED_CPUS
     os.environ.setdefault("OMP_PROC_BIND", "close")
-    os.environ.setdefault("OMP_PLACES", "cores")
-    return "threads bound: OMP_PROC_BIND=%s, OMP_PLACES=%s" % (os.environ["OMP_PROC_BIND"], os.environ["OMP_PLACES"])
+    os.environ.setdefault("OMP_PLACES", ",".join("{%d}" % c for c in cpus))
+    return "threads bound: OMP_PROC_BIND=%s over %d CPUs of the affinity mask" % (os.environ["OMP_PROC_BIND"], len(cpus))
 
 
 def cpu_baseline(S, pbs, prims, steps, nthreads=1):
@@ -361,6 +374,7 @@ def cpu_baseline(S, pbs, prims, steps, nthreads=1):
         _, _, el = oracle.run(op, oracle.REF32, prims[k], n, nthreads=nthreads, timed=True)
         total += pb.nx * n
         secs += el
+    restore_affinity()  # the OpenMP runtime bound this thread to its first place
     how = "1 thread" if nthreads == 1 else "%d OpenMP threads (%s), one worksharing loop per pass" % (nthreads, bound)
     return {"value": total / secs, "unit": "cell-updates/s", "cores": nthreads, "kind": "port",
             "sample": "%d tubes (every 256th of the batch, N=%d..%d), min(steps,2000) steps each, "
@@ -407,7 +421,7 @@ def reference_cpu(cores):
             for _ in range(1 if nth == 1 else 3):
                 with tempfile.TemporaryDirectory() as td:
                     r = subprocess.run([path, "time", str(steps), str(nth)], cwd=td, capture_output=True,
-                                       text=True, timeout=600)
+                                       text=True, timeout=600, preexec_fn=restore_affinity)
                 m = re.search(r"cells (\d+) steps (\d+) threads (\d+) seconds ([0-9.eE+-]+)", r.stdout)
                 if r.returncode != 0 or not m:
                     log("reference timing %s failed: %s" % (exe, r.stderr.strip()[-200:]))
@@ -565,6 +579,7 @@ def cpu_baseline_2d(pbs, prims, nthreads=1, steps=40):
         _, _, el = oracle.run(op, oracle.REF32, prims[k], steps, nthreads=nthreads, timed=True)
         total += pb.nx * pb.ny * steps
         secs += el
+    restore_affinity()
     how = "1 thread" if nthreads == 1 else "%d OpenMP threads (%s), one worksharing loop per pass" % (nthreads, bound)
     return {"value": total / secs, "unit": "cell-updates/s", "cores": nthreads, "kind": "port",
             "sample": "%d domains (every 8th of the batch), %d steps each, %.3g cell-updates in %.1f s; oracle "
