@@ -295,6 +295,14 @@ int main(int argc, char **argv)
         strcpy(o.ic, o.pb.dim == 1 ? "sod"
                      : (o.pb.order == 1 ? "implosion"
                                         : (o.pb.limiter == SHLL_LIM_MC ? "four-contacts" : "four-shocks")));
+    /* the library's limits (shll_init rejects the rest), checked before the host buffer of
+     * nvar*nx*ny doubles is sized from them */
+    if (o.pb.nx < 2 || o.pb.nx > (1LL << 30) ||
+        (o.pb.dim == 2 && (o.pb.ny < 2 || o.pb.ny > (1LL << 30) || (double)o.pb.nx * (double)o.pb.ny > 4.0e9))) {
+        fprintf(stderr, "grid of %lld x %lld cells out of range (2 to 2^30 a side, at most 4e9 cells)\n",
+                (long long)o.pb.nx, (long long)o.pb.ny);
+        return 2;
+    }
     if (o.pb.dim == 1 && strcmp(o.ic, "sod") != 0) {
         fprintf(stderr, "initial data '%s' is 2D; 1D runs take 'sod'\n", o.ic);
         return 2;

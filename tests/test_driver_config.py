@@ -75,6 +75,10 @@ def test_printed_configuration_reads_back(tmp_path):
     (("--ic", "implosion"), "initial data 'implosion' is 2D"),
     (("--dim", "2", "--ic", "sod"), "initial data 'sod' is 1D"),
     (("--frobnicate", "1"), "unknown option 'frobnicate'"),
+    (("--nx", "1"), "out of range"),
+    (("--nx", "1073741825"), "out of range"),
+    (("--dim", "2", "--nx", "100000", "--ny", "100000"), "out of range"),
+    (("--dim", "2", "--nx", "4294967296", "--ny", "4294967296"), "out of range"),
     (("--nx",), "usage:"),
 ])
 def test_bad_options_are_rejected(args, msg):
