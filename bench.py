@@ -554,6 +554,37 @@ def extra_single(S, cpu=True):
                                                "meets_l1_1e-6": bool(max(errs) <= L1_BAR),
                                                "nonfinite_domains": nonfinite}
     ref.close()
+    # the reference's OpenMP solver's own problem (base-omp/2nd_order_base_shll.c: 2D second
+    # order, MC limiter theta 1.25, four contacts, 1024^2, outflow walls, t = 0.3): REF32 is its
+    # arithmetic bit for bit, so its rate compares with the reference timed on the host
+    n3 = 1024
+    ic = S.ic_four_contacts(n3, n3)
+    sol = {}
+    for name, prec in (("ref32", S.REF32), ("f64", S.F64), ("c32k", S.C32K)):
+        pb = S.problem(2, 2, n3, n3, precision=prec, limiter=S.LIM_MC, theta=1.25)
+        s = S.Solver(pb, ic)
+        s.autotune(256)
+        shape = s.get_launch()
+        s.close()
+        s = S.Solver(pb, ic)
+        s.set_launch(*shape)
+        s.step(8)
+        s.close()
+        s = S.Solver(pb, ic)
+        s.set_launch(*shape)
+        t0 = time.perf_counter()
+        n = s.run_to_time(0.3)
+        dt = time.perf_counter() - t0
+        sol[name] = s
+        out["omp_problem_2d_o2_mc_%s_1024sq" % name] = {"steps": n, "seconds": dt,
+                                                        "cell_updates_per_s": n3 * n3 * n / dt,
+                                                        "launch": list(shape)}
+    for name in ("ref32", "c32k"):
+        e = sol[name].compare(sol["f64"])
+        out["omp_problem_2d_o2_mc_%s_1024sq" % name]["error_vs_f64"] = e
+        out["omp_problem_2d_o2_mc_%s_1024sq" % name]["meets_l1_1e-6"] = bool(e["l1_rho"] <= L1_BAR)
+    for s in sol.values():
+        s.close()
     return out
 
 
@@ -833,6 +864,11 @@ def main():
     if not args.no_extra:
         result["extra"] = {"c3_forms": forms_out}
         result["extra"].update(extra_single(S, cpu=not args.no_cpu))
+        omp = (result.get("cpu_reference") or {}).get("omp_on_2nd_order_base_shll_%dt" % host_cores())
+        gpu = result["extra"].get("omp_problem_2d_o2_mc_ref32_1024sq")
+        if omp and gpu:
+            # the same problem in the same arithmetic: the GPU against the reference's OpenMP
+            gpu["vs_reference_openmp_all_cores"] = gpu["cell_updates_per_s"] / omp["value"]
     print(json.dumps(result))
 
 
