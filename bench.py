@@ -245,7 +245,7 @@ def updates_done(pbs, steps, K):
 # ---------------------------------------------------------------------------
 # accuracy of the tube batch against the bitwise FP64 kernel
 # ---------------------------------------------------------------------------
-class TubeJudge:
+class TubeReferee:
     """The FP64 kernel's run of every tube (bitwise the FP64 oracle), and a second FP64
     run from initial temperatures perturbed by PERTURB (relative): a tube whose two FP64
     runs differ by more than the bar cannot be pinned to 1e-6 by FP64 itself."""
@@ -291,21 +291,21 @@ class TubeJudge:
         self.ref.close()
 
 
-def tube_accuracy(S, b, pbs, judge):
-    """L1 / Linf of rho of every tube of batch b (run to its end) against the FP64 judge."""
+def tube_accuracy(S, b, pbs, referee):
+    """L1 / Linf of rho of every tube of batch b (run to its end) against the FP64 referee."""
     import numpy as np
-    judge.finish()
+    referee.finish()
     bad = b.check_state()
     n = len(pbs)
     l1 = np.full(n, np.nan)
     linf = np.full(n, np.nan)
     for k in range(n):
-        if judge.bad[k] == 0 and bad[k] == 0:
-            e = b.compare(k, judge.ref)
+        if referee.bad[k] == 0 and bad[k] == 0:
+            e = b.compare(k, referee.ref)
             l1[k], linf[k] = e["l1_rho"], e["linf_rho"]
     size = np.array([p.nx for p in pbs])
     ok = np.isfinite(l1)
-    rep = ok & judge.reproducible
+    rep = ok & referee.reproducible
     bands = {}
     for lo, hi in ((256, 1024), (1024, 4096), (4096, 16384), (16384, 65537)):
         m = rep & (size >= lo) & (size < hi)
@@ -314,14 +314,14 @@ def tube_accuracy(S, b, pbs, judge):
                 "tubes": int(m.sum()), "l1_rho_median": float(np.median(l1[m])), "l1_rho_max": float(l1[m].max()),
                 "frac_le_1e-6": float((l1[m] <= L1_BAR).mean())}
     fails = rep & (l1 > L1_BAR)
-    irr = ok & ~judge.reproducible & np.isfinite(judge.self_l1)
-    spread = (l1[irr] / np.maximum(judge.self_l1[irr], 1e-300)) if irr.any() else None
+    irr = ok & ~referee.reproducible & np.isfinite(referee.self_l1)
+    spread = (l1[irr] / np.maximum(referee.self_l1[irr], 1e-300)) if irr.any() else None
     return {
-        "judge": "FP64 kernel (bitwise the FP64 oracle), every tube to t = 0.2",
+        "referee": "FP64 kernel (bitwise the FP64 oracle), every tube to t = 0.2",
         "tubes": n,
-        "tubes_nonfinite_fp64": int((judge.bad > 0).sum()),
-        "tubes_nonfinite_this_form_only": int(((bad > 0) & (judge.bad == 0)).sum()),
-        "tubes_fp64_irreproducible": int((~judge.reproducible & (judge.bad == 0)).sum()),
+        "tubes_nonfinite_fp64": int((referee.bad > 0).sum()),
+        "tubes_nonfinite_this_form_only": int(((bad > 0) & (referee.bad == 0)).sum()),
+        "tubes_fp64_irreproducible": int((~referee.reproducible & (referee.bad == 0)).sum()),
         "irreproducible_rule": "FP64 from T*(1+1e-15) differs from FP64 by L1(rho) > 1e-7 (a tenth of the bar)",
         "reproducible_tubes": int(rep.sum()),
         "l1_rho_median": float(np.median(l1[rep])) if rep.any() else None,
@@ -522,7 +522,7 @@ def extra_single(S, cpu=True):
     for s in sol.values():
         s.close()
     # 64 uneven 2D domains, first order, 1000 steps, both FP32-compensated forms; the FP64
-    # batch of the same domains judges accuracy
+    # batch of the same domains is the accuracy referee
     pbs, prims = S.domain_batch(DOMAIN_SEED, NDOMAINS, precision=S.C32K)
     cells = sum(p.nx * p.ny for p in pbs)
     ref = S.Batch([S.problem(2, 1, p.nx, p.ny, precision=S.F64, lx=p.lx, ly=p.ly) for p in pbs], prims,
@@ -662,7 +662,7 @@ def roofline(upd, launches, ev_ms, avg_ms, B, shape, traffic_summary, valu_summa
 def main_domains(S, args):
     """64 uneven first-order 2D domains (side lengths log-uniform in [256, 2048], 1-4
     circles each), 1000 steps, FP32-compensated, one launch per k steps over the
-    (domain, tile) work list; the FP64 batch of the same domains judges accuracy."""
+    (domain, tile) work list; the FP64 batch of the same domains is the accuracy referee."""
     form = args.form if args.form in ("c32k", "c32", "f64") else "c32k"
     prec = S.PREC_NAMES[form]
     pbs, prims = S.domain_batch(DOMAIN_SEED, NDOMAINS, precision=prec)
@@ -684,7 +684,7 @@ def main_domains(S, args):
         ref.run()
         dt64 = time.perf_counter() - t0
         errs = [b.compare(k, ref)["l1_rho"] / (pbs[k].lx * pbs[k].ly) for k in range(NDOMAINS)]
-        acc = {"judge": "FP64 batch of the same domains, 1000 steps (its launch shape %s, %.3e cell-updates/s)" % (
+        acc = {"referee": "FP64 batch of the same domains, 1000 steps (its launch shape %s, %.3e cell-updates/s)" % (
                    list(ref.get_launch()), updates_done(pbs, steps, DOMAIN_STEPS) / dt64),
                "l1_rho_per_unit_area_max": max(errs), "l1_rho_per_unit_area_median": sorted(errs)[NDOMAINS // 2],
                "meets_l1_1e-6": bool(max(errs) <= L1_BAR),
@@ -723,7 +723,7 @@ def main():
                     help="untimed warm-up time steps (then reset); the default is one 128-step graph chunk")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
-    ap.add_argument("--no-accuracy", action="store_true", help="skip the FP64 judge of the batch")
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the FP64 referee of the batch")
     ap.add_argument("--progress", type=int, default=0,
                     help="advance the timed run in pieces of this many steps and log each (profiling runs)")
     ap.add_argument("--launch", default="", help="C,k,waves,graph (default: the recorded autotuner choice)")
@@ -763,16 +763,16 @@ def main():
                                                                  sum(p.nx for p in pbs), total_steps))
     K = min(args.steps, total_steps) if args.steps > 0 else total_steps
     launch = tuple(int(x) for x in args.launch.split(",")) if args.launch else None
-    judge = acc = None
+    referee = acc = None
     forms_out = {}
-    # the timed batch is built and uploaded first, then the FP64 judge (every tube, twice)
+    # the timed batch is built and uploaded first, then the FP64 referee (every tube, twice)
     # runs, then the W warm-up steps and the timed window follow with no idle gap: the GPU
     # lowers its clocks within 50 ms of idling and takes tens of ms of load to raise them
     b = make_batch(S, pbs, prims, steps, launch)
     if not args.no_accuracy:
         t1 = time.perf_counter()
-        judge = TubeJudge(S, pbs, prims, steps)
-        log("FP64 judge and its perturbed twin in %.1f s" % (time.perf_counter() - t1))
+        referee = TubeReferee(S, pbs, prims, steps)
+        log("FP64 referee and its perturbed twin in %.1f s" % (time.perf_counter() - t1))
     b, wall, avg_ms, launches, ev_ms = run_batch(S, pbs, prims, steps, launch, K, args.warmup, args.progress, b=b)
     upd = updates_done(pbs, steps, K)
     value = upd / wall
@@ -780,15 +780,15 @@ def main():
     lau = list(b.get_launch())
     log("timed %d steps: %.3e cell-updates in %.3f s (events %.1f ms, %d launches)" % (K, upd, wall, ev_ms, launches))
     whole = {"steps": int(K), "seconds": wall, "cell_updates_per_s": value}
-    if K < total_steps and judge is not None:
+    if K < total_steps and referee is not None:
         # the rest of the job, timed apart, for the whole job's rate (window + rest)
         t1 = time.perf_counter()
         b.run()
         rest = time.perf_counter() - t1
         whole = {"steps": int(total_steps), "seconds": wall + rest,
                  "cell_updates_per_s": updates_done(pbs, steps, total_steps) / (wall + rest)}
-    if judge is not None:
-        acc = tube_accuracy(S, b, pbs, judge)
+    if referee is not None:
+        acc = tube_accuracy(S, b, pbs, referee)
         acc["timed_steps_then_run_to_end"] = K < total_steps
     b.close()
     if not args.no_extra:
@@ -796,23 +796,23 @@ def main():
         for name in ("f64f", "c32k", "c32", "f64"):
             if name == args.form:
                 continue
-            if name == "f64" and judge is not None:
+            if name == "f64" and referee is not None:
                 forms_out[name] = {"form": FORMS[name][1], "cell_updates_per_s": updates_done(pbs, steps, total_steps)
-                                   / judge.seconds, "launch_shape": judge.launch,
-                                   "accuracy": "bitwise the FP64 oracle (the judge)"}
+                                   / referee.seconds, "launch_shape": referee.launch,
+                                   "accuracy": "bitwise the FP64 oracle (the referee)"}
                 continue
             pp = [S.problem(1, 2, p.nx, precision=S.PREC_NAMES[name]) for p in pbs]
             bo, wo, ao, lo_, _ = run_batch(S, pp, prims, steps, None, total_steps, args.warmup)
             forms_out[name] = {"form": FORMS[name][1], "cell_updates_per_s": updates_done(pbs, steps, total_steps) / wo,
                                "avg_launch_ms": ao, "launches": lo_, "launch_shape": list(bo.get_launch())}
-            if judge is not None and name != "f64":
-                a = tube_accuracy(S, bo, pbs, judge)
+            if referee is not None and name != "f64":
+                a = tube_accuracy(S, bo, pbs, referee)
                 forms_out[name]["accuracy"] = a
                 forms_out[name]["meets_l1_1e-6_on_reproducible_tubes"] = a["tubes_over_1e-6"] == 0
             log("%s: %.3e cell-updates/s" % (name, forms_out[name]["cell_updates_per_s"]))
             bo.close()
-    if judge is not None:
-        judge.close()
+    if referee is not None:
+        referee.close()
     kern = "shll::k_step1d<%d, 2, %d, %d, 0> (%s, second order minmod, %d cells/lane, %d waves/block)" % (
         prec, lau[0], lau[2], FORMS[args.form][1], lau[0], lau[2])
     result = {
@@ -837,9 +837,9 @@ def main():
                              "graph_steps": lau[3]},
         },
         "whole_job": dict(whole, note=("the timed window of K steps, then the rest of the job to t = 0.2 (every "
-                                       "tube to its own step count) timed apart; the FP64 accuracy judge runs "
+                                       "tube to its own step count) timed apart; the FP64 accuracy referee runs "
                                        "before both") if whole["steps"] == total_steps and K < total_steps else
-                          "the timed window (the rest of the job is not run without the accuracy judge)"
+                          "the timed window (the rest of the job is not run without the accuracy referee)"
                           if K < total_steps else "the timed window is the whole job"),
         "roofline": roofline(upd, launches, ev_ms, avg_ms, B, lau,
                              "k_step1d_%s_o2_tubes4096_hbm_traffic.md" % args.form,

@@ -27,12 +27,12 @@
  *   ORC_C32   : float flux arithmetic with a compensated (hi, lo) float state
  *               (TwoSum update).  Restates the product's FP32-compensated
  *               kernels so they can be checked bitwise; its accuracy is
- *               judged against ORC_F64.
+ *               measured against ORC_F64.
  *   ORC_C32K  : ORC_C32 with the face terms added to each cell by one Kahan
  *               (compensated summation) step instead of the TwoSum chain.
  *   ORC_F64F  : 1D only: ORC_C32K's formulas with every value a double and the
  *               face terms added to the FP64 state directly (no compensation).
- *               Restates the product's fast FP64 kernel; judged against ORC_F64.
+ *               Restates the product's fast FP64 kernel; measured against ORC_F64.
  *
  * Parity pinning status (which check fixes which function) is listed per
  * function in shll_oracle.c.

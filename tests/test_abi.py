@@ -121,7 +121,7 @@ def test_steps_to_time_reports_a_float_time_loop_that_never_ends(orc):
 
 
 def test_integration_example_compiles_against_the_header(tmp_path):
-    """INTEGRATION.md's gpu_main.c (its first C block) is valid C99 against include/shll.h
+    """The integration guide's gpu_main.c (its first C block) is valid C99 against include/shll.h
     (the GPU test links and runs it with the reference's own base_shll.c)."""
     import subprocess
     text = open(os.path.join(ROOT, "INTEGRATION.md")).read()

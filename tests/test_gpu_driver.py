@@ -125,7 +125,7 @@ def test_momentum_change_equals_wall_impulse_2d():
 @pytest.mark.skipif(not os.path.exists(os.path.join(ROOT, "oracle", "_ref", "integration_base_shll.run")),
                     reason="built only where the reference tree was present at build time")
 def test_integration_example_is_the_reference_file_on_the_gpu(tmp_path):
-    """INTEGRATION.md's gpu_main.c linked with the reference's own base_shll.c (main renamed):
+    """The integration guide's gpu_main.c linked with the reference's own base_shll.c (main renamed):
     the reference's stdout line and results.dat, byte for byte."""
     g = np.load(os.path.join(GOLD, "o1_1d_main.npz"), allow_pickle=False)
     exe = os.path.join(ROOT, "oracle", "_ref", "integration_base_shll.run")
