@@ -46,10 +46,13 @@ second-order solver with OpenMP on 1 core and on every core, and the base-c solv
 configurations' sizes.
 
 --workload domains times the 64 uneven 2D first-order domains instead (same JSON line,
-2D roofline), for profiling the 2D kernel.
+2D roofline), for profiling the 2D kernel; --workload omp2d times the reference OpenMP
+solver's own problem (base-omp/2nd_order_base_shll.c: 2D second order, MC limiter, four
+contacts, 1024^2, t = 0.3), by default in that file's own arithmetic (REF32), beside that file
+timed on the host.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--form f64f|c32k|c32|f64]
-                       [--no-cpu] [--no-extra] [--no-accuracy] [--workload tubes|domains]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--form f64f|c32k|c32|f64|ref32]
+                       [--no-cpu] [--no-extra] [--no-accuracy] [--workload tubes|domains|omp2d]
                        [--launch C,k,waves,graph]
 """
 import argparse
@@ -93,6 +96,8 @@ FORMS = {
     "c32k": ("c32k", "FP32 fluxes + (hi,lo) FP32 state, Kahan-compensated update (SHLL_C32K)", 48, "f32"),
     "c32": ("c32", "FP32 fluxes + (hi,lo) FP32 state, exact TwoSum face update (SHLL_C32)", 48, "f32"),
     "f64": ("f64", "FP64 state and fluxes, reference association (SHLL_F64, bitwise the FP64 oracle)", 48, "f64"),
+    "ref32": ("ref32", "float with the reference's double promotions (SHLL_REF32, bit for bit the compiled reference)",
+              24, "f32"),
 }
 
 
@@ -715,6 +720,78 @@ def main_domains(S, args):
     print(json.dumps(result))
 
 
+def main_omp2d(S, args, ref_cpu):
+    """The reference OpenMP solver's own problem, one instance: base-omp/2nd_order_base_shll.c,
+    2D second order, MC limiter theta = 1.25 (:44), its four contacts (:149-167) on 1024^2
+    (:21-22), outflow walls, to t = 0.3 (:43).  REF32 by default: the file's arithmetic bit
+    for bit, so the rate compares with that file timed on the host (cpu_reference)."""
+    form = args.form if args.form in ("ref32", "f64", "c32k", "c32") else "ref32"
+    prec = S.PREC_NAMES[form]
+    n = 1024
+    pb = S.problem(2, 2, n, n, precision=prec, limiter=S.LIM_MC, theta=1.25)
+    ic = S.ic_four_contacts(n, n)
+    total = S.steps_to_time(pb, 0.3)
+    K = min(args.steps, total) if args.steps > 0 else total
+    s = S.Solver(pb, ic)
+    if args.launch:
+        s.set_launch(*(int(x) for x in args.launch.split(",")))
+    else:
+        s.autotune(256)  # the recorded choice when the tuning cache has one
+    lau = list(s.get_launch())
+    if args.warmup > 0:
+        # warm-up steps (graph capture included), then back to the initial state
+        raw0 = s.state_raw()
+        s.step(args.warmup)
+        s.set_state_raw(raw0)
+    s.set_timing(True)
+    t0 = time.perf_counter()
+    s.step(K)
+    wall = time.perf_counter() - t0
+    avg_ms, launches, ev_ms = s.last_timing()
+    upd = n * n * K
+    log("timed %d steps: %.3e cell-updates in %.3f s (events %.1f ms, %d launches)" % (K, upd, wall, ev_ms, launches))
+    acc = None
+    if not args.no_accuracy and form != "f64":
+        with S.Solver(S.problem(2, 2, n, n, precision=S.F64, limiter=S.LIM_MC, theta=1.25), ic) as r:
+            r.autotune(256)
+            r.step(K)
+            e = s.compare(r)
+        acc = {"referee": "FP64 kernel (bitwise the FP64 oracle), the same %d steps" % K,
+               "l1_rho": e["l1_rho"], "linf_rho": e["linf_rho"], "max_rel_rho": e["max_rel_rho"],
+               "meets_l1_1e-6": bool(e["l1_rho"] <= L1_BAR)}
+    s.close()
+    B = bytes_per_update(2, prec)
+    k = lau[1]
+    kern = ("shll::k_step2d<%d, 2, %d, 1> (second order, MC limiter, one step per launch)" % (prec, lau[2])
+            if k == 1 else "shll::k_step2d_tb2<%d, 2, %d, 1> (second order, MC limiter, 2 steps per launch)"
+            % (prec, lau[2]))
+    tag = "k_step2d_%s_o2_mc1024" % form if k == 1 else "k_step2d_tb2_%s_o2_mc1024" % form
+    result = {
+        "metric": METRIC, "value": upd / wall, "unit": "cell-updates/s", "n_gpus": 1, "steps": K,
+        "warmup": args.warmup, "ms_per_step": wall * 1e3 / K, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": FORMS[form][3], "data": "synthetic",
+        "config": {"workload": "the reference OpenMP solver's problem (2D second order, MC limiter theta 1.25, "
+                               "four contacts, outflow walls), " + FORMS[form][1],
+                   "nx": n, "ny": n, "t_end": 0.3, "steps_to_t_end": int(total), "cell_updates_timed": int(upd),
+                   "launches": launches, "parallelism": "1 GPU",
+                   "launch_shape": {"steps_per_launch": k, "waves_per_block": lau[2], "graph_steps": lau[3]}},
+        "roofline": roofline(upd, launches, ev_ms, avg_ms, B, lau, tag + "_hbm_traffic.md", tag + "_sq_counters.md",
+                             kern),
+        "accuracy": acc,
+    }
+    if k == 1:
+        result["roofline"]["bound"] = "hbm"
+        result["roofline"]["note"] = ("one step per launch: the state is read and written every step. " +
+                                      result["roofline"]["note"].split(": the kernel is bound by VALU issue. ", 1)[-1])
+    if ref_cpu:
+        result["cpu_reference"] = ref_cpu
+        omp = ref_cpu.get("omp_on_2nd_order_base_shll_%dt" % host_cores())
+        if omp and form == "ref32":
+            result["vs_reference_openmp_all_cores"] = result["value"] / omp["value"]
+    result["cpu_baseline"] = (ref_cpu or {}).get("omp_on_2nd_order_base_shll_1t")
+    print(json.dumps(result))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -732,9 +809,10 @@ def main():
     ap.add_argument("--form", default="f64f", choices=sorted(FORMS),
                     help="arithmetic of the timed batch: f64f (FP64 face form, meets L1 1e-6; default), c32k / c32 "
                          "(FP32-compensated), f64 (bitwise FP64)")
-    ap.add_argument("--workload", default="tubes", choices=["tubes", "domains"],
+    ap.add_argument("--workload", default="tubes", choices=["tubes", "domains", "omp2d"],
                     help="tubes: the 4096-tube second-order batch (headline); domains: 64 uneven 2D "
-                         "first-order domains, 1000 steps")
+                         "first-order domains, 1000 steps; omp2d: the reference OpenMP solver's own "
+                         "problem (2D second order, MC limiter, 1024^2, t = 0.3; --form ref32 by default)")
     args = ap.parse_args()
     if not args.autotune:
         os.environ.setdefault("SHLL_TUNE_CACHE", TUNE_CACHE)
@@ -744,7 +822,7 @@ def main():
     # the reference's own solvers are timed first, on a quiet host: inside the full run their
     # 16-thread timing came out 1.7x slower than from a fresh process
     ref_cpu = None
-    if not args.no_cpu and args.workload == "tubes":
+    if not args.no_cpu and args.workload in ("tubes", "omp2d"):
         t0 = time.perf_counter()
         ref_cpu = reference_cpu(host_cores())
         log("reference solvers timed on the host in %.1f s" % (time.perf_counter() - t0))
@@ -754,6 +832,9 @@ def main():
         calibration_copy(args.calib_copy)
     if args.workload == "domains":
         main_domains(S, args)
+        return
+    if args.workload == "omp2d":
+        main_omp2d(S, args, ref_cpu)
         return
     prec = S.PREC_NAMES[args.form]
     t0 = time.perf_counter()
