@@ -183,7 +183,10 @@ shll_status shll_set_launch(shll_ctx *ctx, int32_t cells_per_lane, int32_t steps
 shll_status shll_get_launch(const shll_ctx *ctx, int32_t out[4]);
 /* Time candidate launch shapes on this context's problem (state is restored) and
  * keep the fastest; the choice is cached per (device, kernel, shape class) in
- * $SHLL_TUNE_CACHE (if set). */
+ * $SHLL_TUNE_CACHE (if set).  Candidates: 1D cells per lane 1-8, 1-64 steps per launch,
+ * 1 or 4 waves per block (and 256-4096 steps per launch when the whole tube fits one
+ * segment or block, which then has no halo); 2D the valid steps per launch, 1 or 4
+ * waves per block. */
 shll_status shll_autotune(shll_ctx *ctx, int32_t trial_steps);
 
 /* Average duration (ms) of the step kernel launches of the last shll_step /
