@@ -49,10 +49,11 @@ configurations' sizes.
 2D roofline), for profiling the 2D kernel; --workload omp2d times the reference OpenMP
 solver's own problem (base-omp/2nd_order_base_shll.c: 2D second order, MC limiter, four
 contacts, 1024^2, t = 0.3), by default in that file's own arithmetic (REF32), beside that file
-timed on the host.
+timed on the host; --workload sod65536 times one 65536-cell second-order Sod tube to t = 0.2
+(a single instance: latency-bound).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--form f64f|c32k|c32|f64|ref32]
-                       [--no-cpu] [--no-extra] [--no-accuracy] [--workload tubes|domains|omp2d]
+                       [--no-cpu] [--no-extra] [--no-accuracy] [--workload tubes|domains|omp2d|sod65536]
                        [--launch C,k,waves,graph]
 """
 import argparse
@@ -720,6 +721,64 @@ def main_domains(S, args):
     print(json.dumps(result))
 
 
+def main_sod65536(S, args):
+    """One 65536-cell Sod tube, second order, to t = 0.2 (104858 steps; the reference's 1D
+    Sod data, base_shll.c:54-60, with the outflow walls of 2nd_order_base_shll.c): a single
+    instance of 1e5 dependent steps, bound by the latency of one wavefront per SIMD."""
+    form = args.form if args.form in ("f64f", "f64", "c32k", "c32", "ref32") else "f64f"
+    prec = S.PREC_NAMES[form]
+    n = 65536
+    pb = S.problem(1, 2, n, precision=prec)
+    ic = S.ic_tube(n)
+    total = S.steps_to_time(pb, 0.2)
+    K = min(args.steps, total) if args.steps > 0 else total
+    s = S.Solver(pb, ic)
+    if args.launch:
+        s.set_launch(*(int(x) for x in args.launch.split(",")))
+    else:
+        s.autotune(8000)  # the recorded choice when the tuning cache has one
+    lau = list(s.get_launch())
+    if args.warmup > 0:
+        raw0 = s.state_raw()
+        s.step(args.warmup)
+        s.set_state_raw(raw0)
+    s.set_timing(True)
+    t0 = time.perf_counter()
+    s.step(K)
+    wall = time.perf_counter() - t0
+    avg_ms, launches, ev_ms = s.last_timing()
+    upd = n * K
+    log("timed %d steps: %.3e cell-updates in %.3f s (events %.1f ms, %d launches)" % (K, upd, wall, ev_ms, launches))
+    acc = None
+    if not args.no_accuracy and form != "f64":
+        with S.Solver(S.problem(1, 2, n, precision=S.F64), ic) as r:
+            r.autotune(8000)
+            r.step(K)
+            e = s.compare(r)
+        acc = {"referee": "FP64 kernel (bitwise the FP64 oracle), the same %d steps" % K, "l1_rho": e["l1_rho"],
+               "linf_rho": e["linf_rho"], "meets_l1_1e-6": bool(e["l1_rho"] <= L1_BAR)}
+    s.close()
+    B = bytes_per_update(1, prec)
+    kern = "shll::k_step1d<%d, 2, %d, %d, 0> (%s, %d cells/lane, %d waves/block)" % (
+        prec, lau[0], lau[2], FORMS[form][1], lau[0], lau[2])
+    tag = "k_step1d_%s_o2_sod65536" % form
+    result = {
+        "metric": METRIC, "value": upd / wall, "unit": "cell-updates/s", "n_gpus": 1, "steps": K,
+        "warmup": args.warmup, "ms_per_step": wall * 1e3 / K, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": FORMS[form][3], "data": "synthetic",
+        "config": {"workload": "one 65536-cell Sod tube, second order, outflow walls, t = 0.2, " + FORMS[form][1],
+                   "nx": n, "steps_to_t_end": int(total), "cell_updates_timed": int(upd), "launches": launches,
+                   "parallelism": "1 GPU",
+                   "launch_shape": {"cells_per_lane": lau[0], "steps_per_launch": lau[1], "waves_per_block": lau[2],
+                                    "graph_steps": lau[3]}},
+        "roofline": roofline(upd, launches, ev_ms, avg_ms, B, lau, tag + "_hbm_traffic.md", tag + "_sq_counters.md",
+                             kern),
+        "accuracy": acc,
+        "cpu_baseline": None if args.no_cpu else cpu_rate(S.problem(1, 2, n), ic, 400),
+    }
+    print(json.dumps(result))
+
+
 def main_omp2d(S, args, ref_cpu):
     """The reference OpenMP solver's own problem, one instance: base-omp/2nd_order_base_shll.c,
     2D second order, MC limiter theta = 1.25 (:44), its four contacts (:149-167) on 1024^2
@@ -809,10 +868,11 @@ def main():
     ap.add_argument("--form", default="f64f", choices=sorted(FORMS),
                     help="arithmetic of the timed batch: f64f (FP64 face form, meets L1 1e-6; default), c32k / c32 "
                          "(FP32-compensated), f64 (bitwise FP64)")
-    ap.add_argument("--workload", default="tubes", choices=["tubes", "domains", "omp2d"],
+    ap.add_argument("--workload", default="tubes", choices=["tubes", "domains", "omp2d", "sod65536"],
                     help="tubes: the 4096-tube second-order batch (headline); domains: 64 uneven 2D "
                          "first-order domains, 1000 steps; omp2d: the reference OpenMP solver's own "
-                         "problem (2D second order, MC limiter, 1024^2, t = 0.3; --form ref32 by default)")
+                         "problem (2D second order, MC limiter, 1024^2, t = 0.3; --form ref32 by default); "
+                         "sod65536: one 65536-cell second-order Sod tube to t = 0.2")
     args = ap.parse_args()
     if not args.autotune:
         os.environ.setdefault("SHLL_TUNE_CACHE", TUNE_CACHE)
@@ -835,6 +895,9 @@ def main():
         return
     if args.workload == "omp2d":
         main_omp2d(S, args, ref_cpu)
+        return
+    if args.workload == "sod65536":
+        main_sod65536(S, args)
         return
     prec = S.PREC_NAMES[args.form]
     t0 = time.perf_counter()
