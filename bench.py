@@ -50,10 +50,10 @@ configurations' sizes.
 solver's own problem (base-omp/2nd_order_base_shll.c: 2D second order, MC limiter, four
 contacts, 1024^2, t = 0.3), by default in that file's own arithmetic (REF32), beside that file
 timed on the host; --workload sod65536 times one 65536-cell second-order Sod tube to t = 0.2
-(a single instance: latency-bound).
+(a single instance: latency-bound), --workload circle4096 one 4096^2 first-order 2D domain.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--form f64f|c32k|c32|f64|ref32]
-                       [--no-cpu] [--no-extra] [--no-accuracy] [--workload tubes|domains|omp2d|sod65536]
+                       [--no-cpu] [--no-extra] [--no-accuracy] [--workload tubes|domains|omp2d|sod65536|circle4096]
                        [--launch C,k,waves,graph]
 """
 import argparse
@@ -779,6 +779,63 @@ def main_sod65536(S, args):
     print(json.dumps(result))
 
 
+def main_circle4096(S, args):
+    """One 4096^2 first-order 2D domain (the circle of configuration c2: rho 10 inside r = 0.25,
+    reflective walls, base_shll_2d.c's scheme), 2000 steps."""
+    form = args.form if args.form in ("f64", "c32k", "c32", "ref32") else "c32k"
+    prec = S.PREC_NAMES[form]
+    n = 4096
+    pb = S.problem(2, 1, n, n, precision=prec)
+    ic = S.ic_circle(n, n)
+    total = 2000
+    K = min(args.steps, total) if args.steps > 0 else total
+    s = S.Solver(pb, ic)
+    if args.launch:
+        s.set_launch(*(int(x) for x in args.launch.split(",")))
+    else:
+        s.autotune(256)  # the recorded choice when the tuning cache has one
+    lau = list(s.get_launch())
+    if args.warmup > 0:
+        raw0 = s.state_raw()
+        s.step(args.warmup)
+        s.set_state_raw(raw0)
+    s.set_timing(True)
+    t0 = time.perf_counter()
+    s.step(K)
+    wall = time.perf_counter() - t0
+    avg_ms, launches, ev_ms = s.last_timing()
+    upd = n * n * K
+    log("timed %d steps: %.3e cell-updates in %.3f s (events %.1f ms, %d launches)" % (K, upd, wall, ev_ms, launches))
+    acc = None
+    if not args.no_accuracy and form != "f64":
+        with S.Solver(S.problem(2, 1, n, n, precision=S.F64), ic) as r:
+            r.autotune(256)
+            r.step(K)
+            e = s.compare(r)
+        acc = {"referee": "FP64 kernel (bitwise the FP64 oracle), the same %d steps" % K, "l1_rho": e["l1_rho"],
+               "linf_rho": e["linf_rho"], "meets_l1_1e-6": bool(e["l1_rho"] <= L1_BAR)}
+    s.close()
+    B = bytes_per_update(2, prec)
+    k = lau[1]
+    kern = ("shll::k_step2d_tb<%d, %d, %d> (first order, %d steps per launch)" % (prec, k, lau[2], k)
+            if k > 1 else "shll::k_step2d<%d, 1, %d, 0>" % (prec, lau[2]))
+    tag = ("k_step2d_tb_%s_circle4096" % form) if k > 1 else ("k_step2d_%s_circle4096" % form)
+    result = {
+        "metric": METRIC, "value": upd / wall, "unit": "cell-updates/s", "n_gpus": 1, "steps": K,
+        "warmup": args.warmup, "ms_per_step": wall * 1e3 / K, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": FORMS[form][3], "data": "synthetic",
+        "config": {"workload": "one 4096^2 first-order 2D domain (circle, reflective walls), 2000 steps, " +
+                               FORMS[form][1], "nx": n, "ny": n, "cell_updates_timed": int(upd), "launches": launches,
+                   "parallelism": "1 GPU",
+                   "launch_shape": {"steps_per_launch": k, "waves_per_block": lau[2], "graph_steps": lau[3]}},
+        "roofline": roofline(upd, launches, ev_ms, avg_ms, B, lau, tag + "_hbm_traffic.md", tag + "_sq_counters.md",
+                             kern),
+        "accuracy": acc,
+        "cpu_baseline": None if args.no_cpu else cpu_rate(S.problem(2, 1, n, n), ic, 2),
+    }
+    print(json.dumps(result))
+
+
 def main_omp2d(S, args, ref_cpu):
     """The reference OpenMP solver's own problem, one instance: base-omp/2nd_order_base_shll.c,
     2D second order, MC limiter theta = 1.25 (:44), its four contacts (:149-167) on 1024^2
@@ -868,11 +925,12 @@ def main():
     ap.add_argument("--form", default="f64f", choices=sorted(FORMS),
                     help="arithmetic of the timed batch: f64f (FP64 face form, meets L1 1e-6; default), c32k / c32 "
                          "(FP32-compensated), f64 (bitwise FP64)")
-    ap.add_argument("--workload", default="tubes", choices=["tubes", "domains", "omp2d", "sod65536"],
+    ap.add_argument("--workload", default="tubes", choices=["tubes", "domains", "omp2d", "sod65536", "circle4096"],
                     help="tubes: the 4096-tube second-order batch (headline); domains: 64 uneven 2D "
                          "first-order domains, 1000 steps; omp2d: the reference OpenMP solver's own "
                          "problem (2D second order, MC limiter, 1024^2, t = 0.3; --form ref32 by default); "
-                         "sod65536: one 65536-cell second-order Sod tube to t = 0.2")
+                         "sod65536: one 65536-cell second-order Sod tube to t = 0.2; circle4096: one 4096^2 "
+                         "first-order 2D domain, 2000 steps")
     args = ap.parse_args()
     if not args.autotune:
         os.environ.setdefault("SHLL_TUNE_CACHE", TUNE_CACHE)
@@ -898,6 +956,9 @@ def main():
         return
     if args.workload == "sod65536":
         main_sod65536(S, args)
+        return
+    if args.workload == "circle4096":
+        main_circle4096(S, args)
         return
     prec = S.PREC_NAMES[args.form]
     t0 = time.perf_counter()
