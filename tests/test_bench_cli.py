@@ -22,7 +22,7 @@ def test_options_parse():
     r = run("--help")
     assert r.returncode == 0
     for opt in ("--steps", "--warmup", "--form", "--workload", "--no-cpu", "--no-extra", "--no-accuracy", "--launch",
-                "omp2d", "domains", "f64f", "ref32"):
+                "omp2d", "domains", "sod65536", "circle4096", "f64f", "ref32"):
         assert opt in r.stdout, opt
     r = run("--form", "f65")
     assert r.returncode == 2 and "invalid choice" in r.stderr
