@@ -11,8 +11,8 @@ the timed region, inputs resident in HBM.
 
 Timed form (default --form f64f): the fastest form that meets the accuracy target
 L1(rho) <= 1e-6 against the FP64 kernel -- FP64 state and fluxes in the compensated
-kernels' face-term arrangement (SHLL_F64F).  The FP32-compensated forms (--form c32k, c32)
-run ~1.5 times faster but miss that target on second-order tubes of more than ~1000 cells:
+kernels' face-term arrangement (SHLL_F64F).  The FP32-compensated forms (--form c32k, c32;
+1.25 and 0.95 times its rate) miss that target on second-order tubes of more than ~1000 cells:
 the scheme amplifies round-off about a thousandfold, so FP32 flux rounding grows to
 L1 ~1e-5.  Every form's full run, its accuracy and its rate are in the line (accuracy,
 extra).

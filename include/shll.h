@@ -50,10 +50,12 @@ typedef enum {
     SHLL_C32K = 3,   /* as SHLL_C32 but the face terms enter each cell through one Kahan
                         update (3x fewer update instructions); same accuracy against FP64,
                         conserved sums to ~1e-10 */
-    SHLL_F64F = 4    /* 1D only: FP64 state and fluxes in SHLL_C32K's arrangement (fused
-                        multiply-adds, one reciprocal of rho, face-term update); not the
-                        reference's association, so within FP64 round-off of SHLL_F64 rather
-                        than bitwise (L1(rho) ~1e-12 on second-order tubes) */
+    SHLL_F64F = 4    /* 1D only: FP64 state and fluxes in SHLL_C32K's face-term arrangement
+                        (fused multiply-adds; 1/rho and 1/sqrt(gamma R T) from faithful
+                        Newton sequences in place of divisions and a square root; fluxes
+                        scaled by DT/DX); not the reference's association, so within FP64
+                        round-off of SHLL_F64 rather than bitwise (L1(rho) ~1e-12 on
+                        second-order tubes) */
 } shll_precision;
 
 typedef enum {

@@ -25,7 +25,8 @@
  *       equals the 1D run, L1 convergence to the exact Riemann solution.
  *   C32: no reference exists (it is this project's compensated scheme); pinned by
  *       the same invariants and by its distance to F64 (L1(rho) <= 1e-6).
- *   F64F: no reference exists (the C32 arrangement in double); pinned by the same
+ *   F64F: no reference exists (the compensated modes' face-term arrangement in double,
+ *       1/sqrt(gamma R T) in place of a square root and a division); pinned by the same
  *       invariants (rest state, F+ + F- = F, M = 0 closed form, mirror symmetry,
  *       conservation, wall impulse, exact Riemann convergence) and by its distance to
  *       the F64 mode, which is FP64 round-off only: elementwise relative 1e-13 after
@@ -408,16 +409,20 @@ static int c32_run(const orc_problem *pb, int kahan, const double *prim_in, int6
 }
 
 /* ------------------------------------------------------------------------- */
-/* FP64 face form (F64F, 1D).  The compensated mode's arrangement with every */
-/* value a double: 1/rho once and products by it, fused multiply-adds where   */
-/* a product feeds a sum (fma is correctly rounded in C99/glibc), correctly   */
-/* rounded sqrt and u/a, minmod as the median of (l, r, 0), face terms        */
-/* t = lam*G (second order: fma(lam/2, d, lam*G)) added to the state as       */
-/* u += t_left - t_right.  The initial state is the F64 mode's.               */
+/* FP64 face form (F64F, 1D): the compensated modes' face-term arrangement in   */
+/* double.  Split fluxes scaled by DT/DX (so a face term is G + d/2 and the      */
+/* update u += t_left - t_right); p = (E - (rho u) u/2) R/cv with no E/rho;      */
+/* 1/a = 1/sqrt(gamma R p/rho), M = u/a and a/2 from it (no square root and no   */
+/* division besides 1/rho and 1/sqrt); fused multiply-adds where a product       */
+/* feeds a sum (fma is correctly rounded in C99/glibc); minmod as the median of  */
+/* (l, r, 0).  Here 1/rho and 1/sqrt are 1.0/r and 1.0/sqrt(x); the kernels use  */
+/* a hardware seed and two Newton steps for each (the same 1/rho, and 1/sqrt     */
+/* within 2 ulp, measured), so kernel and oracle agree to round-off, not bit for */
+/* bit.  The initial state is the F64 mode's.                                    */
 /* ------------------------------------------------------------------------- */
 typedef struct {
     int nx, n, order, bc, lim, ke;
-    double gR, rcv, cv, lamx, hlamx, theta;
+    double gR, rcv, cv, lamx, theta;
     double *u[3], *fp[3], *fm[3], *a;
     void *block;
 } f64fgrid;
@@ -445,7 +450,6 @@ static int f64f_alloc(f64fgrid *g, const orc_problem *pb, const double *prim)
     g->cv = k.cv;
     g->rcv = 1.0 / k.cv;
     g->lamx = k.dtdx;
-    g->hlamx = 0.5 * k.dtdx;
     g->block = malloc(n * sizeof(double) * 10 + 64);
     if (!g->block) return -1;
     base = (double *)g->block;
@@ -464,20 +468,21 @@ static int f64f_alloc(f64fgrid *g, const orc_problem *pb, const double *prim)
     return 0;
 }
 
-/* split fluxes of one cell (base_shll.c:135-157 in the F64F arrangement) */
-static void f64f_cell_flux(f64fgrid *g, int c)
+/* split fluxes of one cell scaled by sc (base_shll.c:135-157 in the F64F
+ * arrangement; the step passes DT/DX, the stage entry 1) and the sound speed */
+static void f64f_cell_flux(f64fgrid *g, int c, double sc)
 {
     int v;
     double r = g->u[0][c], m = g->u[1][c], E = g->u[2][c];
     double inv = 1.0 / r;
-    double vx = m * inv, e = E * inv;
-    double T = fma(-0.5 * vx, vx, e) * g->rcv;
-    double a = sqrt(g->gR * T);
-    double M = vx / a;
-    double z1 = fma(0.5, M, 0.5), z2 = (0.5 * a) * fma(-M, M, 1.0), z3 = fma(0.5, M, -0.5);
-    double P = r * T;
+    double vx = m * inv;
+    double P = fma(-0.5 * m, vx, E) * g->rcv; /* rho R T, R = 1 */
+    double x = g->gR * (P * inv);            /* a^2 */
+    double y = 1.0 / sqrt(x);                /* 1/a */
+    double M = vx * y, hs = 0.5 * sc;
+    double z1 = fma(hs, M, hs), z3 = fma(hs, M, -hs), z2 = ((hs * x) * y) * fma(-M, M, 1.0);
     double f[3], U[3];
-    g->a[c] = a;
+    g->a[c] = x * y;
     f[0] = m; f[1] = fma(m, vx, P); f[2] = vx * (E + P);
     U[0] = r; U[1] = m; U[2] = E;
     for (v = 0; v < 3; v++) {
@@ -496,7 +501,7 @@ static double f64f_slope(const f64fgrid *g, const double *q, int s)
     return mm_f64f(0.5 * (q[s + 1] - q[s - 1]), g->theta * m);
 }
 
-/* face term of face k (between cells k-1 and k), as c32_face in double */
+/* face term of face k (between cells k-1 and k) from the scaled split fluxes */
 static double f64f_face(const f64fgrid *g, int v, int k)
 {
     static const double sgn[3] = {-1.0, 1.0, -1.0};
@@ -511,13 +516,13 @@ static double f64f_face(const f64fgrid *g, int v, int k)
         G = p[k - 1] + m[k];
         if (g->order == 2) d = f64f_slope(g, p, k - 1) - f64f_slope(g, m, k);
     }
-    return g->order == 2 ? fma(g->hlamx, d, g->lamx * G) : g->lamx * G;
+    return g->order == 2 ? fma(0.5, d, G) : G;
 }
 
 static void f64f_step(f64fgrid *g)
 {
     int c, v;
-    for (c = 0; c < g->n; c++) f64f_cell_flux(g, c);
+    for (c = 0; c < g->n; c++) f64f_cell_flux(g, c, g->lamx);
     for (v = 0; v < 3; v++) {
         double tl = f64f_face(g, v, 0);
         for (c = 0; c < g->n; c++) {
@@ -650,7 +655,7 @@ int orc_stage(const orc_problem *pb, int mode, const double *prim_in, double *co
         f64fgrid g;
         int v, c;
         if (f64f_alloc(&g, pb, prim_in)) return -2;
-        for (c = 0; c < g.n; c++) f64f_cell_flux(&g, c);
+        for (c = 0; c < g.n; c++) f64f_cell_flux(&g, c, 1.0);
         for (v = 0; v < 3; v++)
             for (c = 0; c < g.n; c++) {
                 size_t o = (size_t)v * g.n + c;

@@ -30,9 +30,12 @@
  *               measured against ORC_F64.
  *   ORC_C32K  : ORC_C32 with the face terms added to each cell by one Kahan
  *               (compensated summation) step instead of the TwoSum chain.
- *   ORC_F64F  : 1D only: ORC_C32K's formulas with every value a double and the
- *               face terms added to the FP64 state directly (no compensation).
- *               Restates the product's fast FP64 kernel; measured against ORC_F64.
+ *   ORC_F64F  : 1D only: the compensated modes' face-term arrangement in double
+ *               (split fluxes scaled by DT/DX, 1/sqrt(gamma R T) in place of a
+ *               square root and a division), the face terms added to the FP64
+ *               state directly.  Restates the product's fast FP64 kernel, which
+ *               agrees with it to round-off (its 1/sqrt is faithful, not correctly
+ *               rounded); measured against ORC_F64.
  *
  * Parity pinning status (which check fixes which function) is listed per
  * function in shll_oracle.c.

@@ -8,12 +8,12 @@
 //                with TwoSum (the FP32-compensated form)
 //   PREC_C32K  : PREC_C32 with one Kahan step per cell for the update
 //   PREC_F64F  : FP64 values in the compensated forms' arrangement (fused multiply-adds,
-//                1/rho once, face-term update); 1D
+//                1/rho and 1/a from faithful Newton sequences, face-term update); 1D
 //
 // The library is compiled with -ffp-contract=off: every + - * below is one IEEE-754
 // operation in source order (fused multiply-adds only where written), and division
 // and square root are correctly rounded (div_rn, sqrt_rn), so the F64 and REF32
-// kernels reproduce the CPU results bit for bit.
+// kernels reproduce the CPU results bit for bit (F64F alone uses faithful sequences).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -169,6 +169,28 @@ __device__ __forceinline__ double sqrt_rn(double x) {
     return __builtin_expect(!(x >= 0x1p-760), 0) ? sqrt(x) : sqrt_fast(x);
 }
 
+// Faithful FP64 reciprocal and reciprocal square root (the F64F form): the hardware seeds
+// (v_rcp_f64, v_rsq_f64; relative error about 2^-24, measured) and two Newton steps each.
+//   rcp_nr2(b): r <- r + r(1 - b r); equal to the correctly rounded 1/b on each of 8e8
+//               random arguments with exponents -1000..1000 (MI355X)
+//   rsq_nr2(x): y <- y (3/2 - (x/2) y^2); within 2 ulp of 1.0/sqrt(x) on the same arguments
+// Neither is correctly rounded by construction, and the seeds are the hardware's, so the
+// F64F results are reproduced on a CPU to within round-off, not bit for bit.  A zero argument
+// gives NaN (not inf); negative and NaN arguments give NaN.
+__device__ __forceinline__ double rcp_nr2(double b) {
+    double r = __builtin_amdgcn_rcp(b);
+    double e = __builtin_fma(-b, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-b, r, 1.0);
+    return __builtin_fma(r, e, r);
+}
+__device__ __forceinline__ double rsq_nr2(double x) {
+    const double nhx = -0.5 * x;
+    double y = __builtin_amdgcn_rsq(x);
+    y = y * __builtin_fma(nhx, y * y, 1.5);
+    return y * __builtin_fma(nhx, y * y, 1.5);
+}
+
 // ---------------------------------------------------------------------------
 // Reference association (F64 and REF32).  RT is the storage/arithmetic type;
 // (double) conversions sit where the reference's double literals and its double
@@ -247,26 +269,9 @@ struct C32Arith {
     __device__ static __forceinline__ float half(float x) { return 0.5f * x; }
 };
 
-// FP64 in the compensated scheme's arrangement (PREC_F64F): the C32 formulas with every
-// value a double -- fused multiply-adds where a product feeds a sum, 1/rho once (correctly
-// rounded) and products by it, correctly rounded sqrt and u/a.  Not the reference's
-// association, so not bitwise the FP64 oracle; its distance to it is FP64 round-off grown
-// by the scheme (about 1e-12 in L1 on the second-order tubes).
+// FP64 in the compensated scheme's arrangement (PREC_F64F): the limiter of the C32 forms in
+// double (the split fluxes are flux1_f64f in shll_flux.h)
 struct F64FArith {
-    __device__ static __forceinline__ void prim1(double u0, double u1, double u2, double rcv, double gR,
-                                                 double& vx, double& T, double& a) {
-        const double inv = div_fast(1.0, u0);
-        vx = u1 * inv;
-        const double e = u2 * inv;
-        T = __builtin_fma(-0.5 * vx, vx, e) * rcv;  // (e - vx^2/2) / cv
-        a = sqrt_fast(gR * T);
-    }
-    // z1, z3 = RN(M/2 +- 1/2) (M/2 is exact), z2 = (a/2)(1 - M^2) with 1 - M^2 fused
-    __device__ static __forceinline__ void zinv(double M, double a, double& z1, double& z2, double& z3) {
-        z1 = __builtin_fma(0.5, M, 0.5);
-        z2 = (0.5 * a) * __builtin_fma(-M, M, 1.0);
-        z3 = __builtin_fma(0.5, M, -0.5);
-    }
     // minmod as the median of (l, r, 0): the smaller magnitude when the signs agree, zero
     // otherwise (sign of a zero result unspecified, as in C32Arith::mm)
     __device__ static __forceinline__ double mm(double l, double r) {

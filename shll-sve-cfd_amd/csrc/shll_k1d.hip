@@ -409,20 +409,22 @@ __device__ __forceinline__ void step_c32(float (&hi)[3][C], float (&lo)[3][C], c
 
 // ---------------------------------------------------------------------------
 // F64F: the C32 face-term arrangement with every value a double (no compensation: the
-// face terms are added to the FP64 state directly, u += t[m] - t[m+1])
+// face terms are added to the FP64 state directly, u += t[m] - t[m+1]).  The split fluxes
+// come scaled by DT/DX, so a face term is G + d/2 (G = F+ of the left cell + F- of the
+// right cell, d = the difference of their limited slopes).
 // ---------------------------------------------------------------------------
 template <int ORDER, int C, int WM, int LIM, int XW>
 __device__ __forceinline__ void step_f64f(double (&u)[3][C], const Inst& in, int gl, const Coop<double, XW>& x,
                                           int parity) {
     using A = F64FArith;
-    const double lam = in.k.d_lamx, hlam = 0.5 * lam, th = in.k.d_theta;
+    const double lam = in.k.d_lamx, th = in.k.d_theta;
     const int n = in.nx;
     if constexpr (WM != WM_NONE) ghost_cells<C>(u, gl, n, in.bc == BC_REFLECT);
     double fp[3][C], fm[3][C];
 #pragma unroll
     for (int m = 0; m < C; ++m) {
         double p[3], q[3], a;
-        flux1<PREC_F64F, double>(u[0][m], u[1][m], u[2][m], in, p, q, a);
+        flux1_f64f(u[0][m], u[1][m], u[2][m], in, lam, p, q, a);  // scaled by DT/DX
 #pragma unroll
         for (int v = 0; v < 3; ++v) {
             fp[v][m] = p[v];
@@ -436,7 +438,7 @@ __device__ __forceinline__ void step_f64f(double (&u)[3][C], const Inst& in, int
             double t[C + 1];
 #pragma unroll
             for (int f = 0; f <= C; ++f)  // face f lies between cells gl+f-1 and gl+f
-                t[f] = __builtin_fma(hlam, w.sp(f) - w.sm(f), lam * (w.FP[f] + w.FM[f + 1]));  // lam*(G + d/2)
+                t[f] = __builtin_fma(0.5, w.sp(f) - w.sm(f), w.FP[f] + w.FM[f + 1]);  // G + d/2
 #pragma unroll
             for (int m = 0; m < C; ++m) u[v][m] = u[v][m] + (t[m] - t[m + 1]);
         }
@@ -462,9 +464,9 @@ __device__ __forceinline__ void step_f64f(double (&u)[3][C], const Inst& in, int
             if constexpr (ORDER == 2) {
                 const double d =
                     slope_at<A, WM, LIM>(FP, f - 1, gf - 1, n, th) - slope_at<A, WM, LIM>(FM, f, gf, n, th);
-                t[f] = __builtin_fma(hlam, d, lam * G);  // lam*(G + d/2)
+                t[f] = __builtin_fma(0.5, d, G);  // G + d/2
             } else {
-                t[f] = lam * G;
+                t[f] = G;
             }
         }
 #pragma unroll

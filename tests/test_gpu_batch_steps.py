@@ -1,13 +1,14 @@
 """GPU: a batch advances by exactly the steps asked for.  shll_batch_advance(n) with n not
 a multiple of the launch's steps ends in a shorter launch; graph chunks that follow start
 at any step.  After every call each instance equals its own oracle run at exactly
-min(steps taken, its step count); the F64F and C32K batches are also checked at the launch
+min(steps taken, its step count) (bitwise; F64F within round-off, as in test_gpu_parity);
+the F64F and C32K batches are also checked at the launch
 shapes the benchmark times (8 cells per lane, 8 or 16 steps per launch, one wavefront per block,
 128-step graphs) with multi-segment tubes and full graph chunks."""
-import numpy as np
 import pytest
 
 import shll_amd as S
+from test_gpu_parity import assert_oracle_match
 
 pytestmark = pytest.mark.gpu
 
@@ -27,7 +28,7 @@ def check_all(orc, b, pbs, prims, steps, done, what):
     for k in range(len(pbs)):
         n = min(done, steps[k])
         _, ref = orc.run(opb(orc, pbs[k]), MODES[pbs[k].precision], prims[k], n, nthreads=4)
-        np.testing.assert_array_equal(b.conserved(k), ref, err_msg="%s: instance %d after %d steps" % (what, k, n))
+        assert_oracle_match(pbs[k].precision, b.conserved(k), ref, "%s: instance %d after %d steps" % (what, k, n))
 
 
 @pytest.mark.parametrize("prec", [S.F64, S.C32K])

@@ -1,7 +1,9 @@
 """GPU parity: the HIP kernels (through the C ABI) against the CPU oracle on the same
 inputs.  F64 and REF32 are compared bitwise (the oracle's reference association and
 IEEE rounding are reproduced exactly); C32 is compared bitwise against the oracle's
-restatement of the compensated scheme and by its distance to F64."""
+restatement of the compensated scheme and by its distance to F64; F64F, whose kernels
+take 1/sqrt(gamma R T) from a hardware seed and two Newton steps (within 2 ulp) where
+the oracle rounds correctly, within a stated round-off tolerance (assert_oracle_match)."""
 import os
 
 import numpy as np
@@ -43,7 +45,37 @@ def oracle_run(orc, pb, prim, nsteps):
     return p, c, raw
 
 
-def assert_same(pb, gpu, ora, what):
+# F64F against the oracle: the largest difference of any value, relative to the largest
+# magnitude of its variable in the oracle's array.  Tolerances are about 8-10x above the
+# largest difference the MI355X measured (every comparison prints its own); a wrong index,
+# sign or dropped term in the kernels shows as 1e-4 or more.
+F64F_TOL_SHORT = 5e-14  # up to 310 steps (measured 6.0e-15)
+F64F_TOL_LONG = 3e-13  # 548 to 2500 steps (measured 3.4e-14)
+F64F_TOL_STAGE = 2e-15  # split fluxes and sound speed of one state (measured 2.7e-16)
+
+
+def scaled_err(got, want):
+    g = np.asarray(got, dtype=np.float64)
+    w = np.asarray(want, dtype=np.float64).reshape(g.shape)
+    if g.ndim == 1:
+        g, w = g[None, :], w[None, :]
+    g, w = g.reshape(g.shape[0], -1), w.reshape(w.shape[0], -1)
+    scale = np.abs(w).max(axis=1)
+    scale[scale == 0] = 1.0
+    return float((np.abs(g - w).max(axis=1) / scale).max())
+
+
+def assert_oracle_match(prec, got, want, what, tol=F64F_TOL_SHORT):
+    """Bitwise for every form but F64F; F64F within tol of the oracle (scaled_err)."""
+    if prec != S.F64F:
+        np.testing.assert_array_equal(got, want, err_msg=what)
+        return
+    e = scaled_err(got, want)
+    print("[f64f-vs-oracle] %s: %.2e" % (what, e), flush=True)
+    assert e <= tol, "%s: F64F is %.3e from the oracle (scaled), above %.0e" % (what, e, tol)
+
+
+def assert_same(pb, gpu, ora, what, tol=F64F_TOL_SHORT):
     gp, gc, graw = gpu
     op, oc, oraw = ora
     # compare physical states only: unstable data (CFL > 1) ends in NaN, which says nothing
@@ -51,8 +83,8 @@ def assert_same(pb, gpu, ora, what):
     if pb.precision in COMP:
         np.testing.assert_array_equal(graw[0], oraw[0], err_msg=what + " hi")
         np.testing.assert_array_equal(graw[1], oraw[1], err_msg=what + " lo")
-    np.testing.assert_array_equal(gc, oc, err_msg=what + " conserved")
-    np.testing.assert_array_equal(gp, op, err_msg=what + " primitives")
+    assert_oracle_match(pb.precision, gc, oc, what + " conserved", tol)
+    assert_oracle_match(pb.precision, gp, op, what + " primitives", tol)
 
 
 @pytest.mark.parametrize("prec", [S.F64, S.REF32, S.C32, S.C32K])
@@ -99,16 +131,21 @@ def test_1d_long_launches_without_halo(orc, prec):
     """A tube that fits one segment (or one cooperative block) has no halo, so any number of
     steps fits one launch: Sod at 1024 cells to t = 0.2 in one 1639-step launch (8 cells per
     lane, a block of 4 wavefronts), and a 500-cell tube in launches of 1024 steps (one
-    wavefront), both walls; bitwise."""
+    wavefront), both walls; bitwise (F64F: bitwise its own run at one step per launch, and
+    within round-off of the oracle)."""
+    def check(pb, ic, n, launch, what):
+        got = gpu_run(pb, ic, n, launch)
+        assert_same(pb, got, oracle_run(orc, pb, ic, n), what, F64F_TOL_LONG)
+        if prec == S.F64F:
+            np.testing.assert_array_equal(got[1], gpu_run(pb, ic, n, (1, 1, 1, 0))[1], err_msg=what)
+
     pb = S.problem(1, 1, 1024, precision=prec)
-    ic = S.ic_tube(1024)
     banner("long launch 1024", prec)
-    assert_same(pb, gpu_run(pb, ic, 1639, (8, 4096, 4, 8192)), oracle_run(orc, pb, ic, 1639), "1639-step launch")
+    check(pb, S.ic_tube(1024), 1639, (8, 4096, 4, 8192), "1639-step launch")
     for bc in (0, 1):
         pb = S.problem(1, 2, 500, precision=prec, bc=bc)
-        ic = icgen.random_1d(500, 82, block=20)
         banner("long launch 500", prec, bc)
-        assert_same(pb, gpu_run(pb, ic, 2500, (8, 1024, 1, 0)), oracle_run(orc, pb, ic, 2500), "1024-step launches")
+        check(pb, icgen.random_1d(500, 82, block=20), 2500, (8, 1024, 1, 0), "1024-step launches")
 
 
 def test_1d_impossible_launch_shape_is_rejected():
