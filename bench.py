@@ -111,6 +111,12 @@ def bytes_per_update(dim, prec):
     return 2 * nv * s
 
 
+def rows_per_item(lau):
+    """2D: owned rows per work item of a launch shape (its first value; 0 or 1: the
+    default for its steps per launch)."""
+    return lau[0] if lau[0] > 1 else {1: 16, 2: 32}.get(lau[1], 64)
+
+
 def profile_figure(summary, key, shape):
     """A per-cell-update figure ("<key>: <value> (launch shape C,k,w)") from a committed
     rocprofv3 summary, when it was taken at the same launch shape; else None."""
@@ -707,7 +713,8 @@ def main_domains(S, args):
                    "domains": NDOMAINS, "seed": DOMAIN_SEED, "side_range": [256, 2048],
                    "cells": int(sum(p.nx * p.ny for p in pbs)), "cell_updates_timed": int(upd),
                    "launches": launches, "parallelism": "1 GPU",
-                   "launch_shape": {"steps_per_launch": lau[1], "waves_per_block": lau[2], "graph_steps": lau[3]}},
+                   "launch_shape": {"rows_per_item": rows_per_item(lau), "steps_per_launch": lau[1],
+                                    "waves_per_block": lau[2], "graph_steps": lau[3]}},
         "roofline": roofline(upd, launches, ev_ms, avg_ms, B, lau,
                              "k_step2d_tb_%s_domains64_hbm_traffic.md" % form,
                              "k_step2d_tb_%s_domains64_sq_counters.md" % form, kern),
@@ -827,7 +834,8 @@ def main_circle4096(S, args):
         "config": {"workload": "one 4096^2 first-order 2D domain (circle, reflective walls), 2000 steps, " +
                                FORMS[form][1], "nx": n, "ny": n, "cell_updates_timed": int(upd), "launches": launches,
                    "parallelism": "1 GPU",
-                   "launch_shape": {"steps_per_launch": k, "waves_per_block": lau[2], "graph_steps": lau[3]}},
+                   "launch_shape": {"rows_per_item": rows_per_item(lau), "steps_per_launch": k,
+                                    "waves_per_block": lau[2], "graph_steps": lau[3]}},
         "roofline": roofline(upd, launches, ev_ms, avg_ms, B, lau, tag + "_hbm_traffic.md", tag + "_sq_counters.md",
                              kern),
         "accuracy": acc,
@@ -890,7 +898,8 @@ def main_omp2d(S, args, ref_cpu):
                                "four contacts, outflow walls), " + FORMS[form][1],
                    "nx": n, "ny": n, "t_end": 0.3, "steps_to_t_end": int(total), "cell_updates_timed": int(upd),
                    "launches": launches, "parallelism": "1 GPU",
-                   "launch_shape": {"steps_per_launch": k, "waves_per_block": lau[2], "graph_steps": lau[3]}},
+                   "launch_shape": {"rows_per_item": rows_per_item(lau), "steps_per_launch": k,
+                                    "waves_per_block": lau[2], "graph_steps": lau[3]}},
         "roofline": roofline(upd, launches, ev_ms, avg_ms, B, lau, tag + "_hbm_traffic.md", tag + "_sq_counters.md",
                              kern),
         "accuracy": acc,

@@ -173,12 +173,13 @@ shll_status shll_compare_host(shll_ctx *ctx, const double *ref_prim_soa, shll_er
 /* Replaces Free_Memory (base_shll.c:63-72). */
 void shll_free(shll_ctx *ctx);
 
-/* Launch shape of the step kernels.  cells_per_lane in {1,2,4,8} (1D only),
- * steps_per_launch >= 1 (temporal blocking; 2D: 1, 2, 3, 4 or 8 in first order; 1, or 2 for
- * SHLL_C32/SHLL_C32K in second order),
+/* Launch shape of the step kernels.  cells_per_lane: 1D in {1,2,4,8}; 2D the owned rows
+ * of a work item, the cells of its column each lane owns (2-4096; 0 or 1: 16, 32 or 64
+ * rows for 1, 2 or more steps per launch), steps_per_launch >= 1 (temporal blocking; 2D:
+ * 1, 2, 3, 4 or 8 in first order; 1, or 2 for SHLL_C32/SHLL_C32K in second order),
  * waves_per_block in {1,4} (1D, 4 with 2+ cells per lane: the block's wavefronts hold
  * one segment and share their edge fluxes), graph_steps: steps captured per HIP graph
- * (0 = no graph).
+ * (0 = no graph).  shll_get_launch returns the values as set.
  * Results do not depend on the launch shape (bitwise). */
 shll_status shll_set_launch(shll_ctx *ctx, int32_t cells_per_lane, int32_t steps_per_launch,
                             int32_t waves_per_block, int32_t graph_steps);
@@ -188,7 +189,8 @@ shll_status shll_get_launch(const shll_ctx *ctx, int32_t out[4]);
  * $SHLL_TUNE_CACHE (if set).  Candidates: 1D cells per lane 1-8, 1-64 steps per launch,
  * 1 or 4 waves per block (and 256-4096 steps per launch when the whole tube fits one
  * segment or block, which then has no halo); 2D the valid steps per launch, 1 or 4
- * waves per block. */
+ * waves per block, and shorter work items (3/4 down to 1/8 of the default rows) when the
+ * default ones give fewer than 4 wavefronts per SIMD. */
 shll_status shll_autotune(shll_ctx *ctx, int32_t trial_steps);
 
 /* Average duration (ms) of the step kernel launches of the last shll_step /

@@ -173,9 +173,12 @@ struct Launch {
     int rows = 16;      // owned rows per 2D item
 };
 
-// owned rows per 2D item for k steps per launch: an item recomputes k halo rows on each
-// side, so longer items waste less; 16 rows at one step per launch
+// Default owned rows per 2D item for k steps per launch: an item recomputes order*k halo
+// rows on each side, so longer items waste less; 16 rows at one step per launch.  In 2D the
+// launch's cells_per_lane sets the rows instead (each lane owns that many cells of its
+// column); 0 or 1 selects this default.
 static int rows2d(int k) { return k <= 1 ? 16 : (k == 2 ? 32 : 64); }
+static constexpr int kMaxRows2d = 4096;
 
 // 1D tiling of one instance of nx cells: segments of XW*64*C cells (XW wavefronts of a
 // cooperative block, else 1), halo order*k.  The segment that holds a wall also holds
@@ -1125,6 +1128,8 @@ static shll_status set_launch(Engine* E, int32_t C, int32_t k, int32_t wpb, int3
     if (E->dim == 1 && !step1d_supported(C, wpb))
         return fail(SHLL_EINVAL, "unsupported launch shape C=%d waves/block=%d", C, wpb);
     if (E->dim == 2 && wpb != 1 && wpb != 4) return fail(SHLL_EINVAL, "waves/block must be 1 or 4");
+    if (E->dim == 2 && (C < 0 || C > kMaxRows2d))
+        return fail(SHLL_EINVAL, "2D rows per item (cells_per_lane) must be 0-%d (got %d)", kMaxRows2d, C);
     if (k < 1 || graph < 0) return fail(SHLL_EINVAL, "steps per launch must be >= 1, graph steps >= 0");
     if (E->dim == 2 && !step2d_supported(E->prec, E->order, k))
         return fail(SHLL_EINVAL, "2D steps per launch: 1, 2, 3, 4 or 8 in first order; 1, or 2 in the "
@@ -1132,9 +1137,9 @@ static shll_status set_launch(Engine* E, int32_t C, int32_t k, int32_t wpb, int3
     if (E->batch && E->launches_done != 0)
         return fail(SHLL_ESTATE, "a batch's launch shape can only change before it advances (reset first)");
     Launch old = E->L;
-    E->L.C = E->dim == 1 ? C : 1;
+    E->L.C = E->dim == 1 ? C : (C <= 1 ? 1 : C);
     E->L.k = k;
-    if (E->dim == 2) E->L.rows = rows2d(k);
+    if (E->dim == 2) E->L.rows = C <= 1 ? rows2d(k) : C;
     E->L.wpb = wpb;
     E->L.graph = graph;
     E->items_valid = false;
@@ -1224,6 +1229,25 @@ static void tune_cache_store(const std::string& key, const Launch& L) {
     std::fclose(f);
 }
 
+// 2D work items of every instance at k steps per launch and `rows` owned rows per item
+// (the counts tile2d produces)
+static int64_t items2d(const Engine* E, int k, int rows) {
+    const int64_t H = (int64_t)E->order * k;
+    int64_t n = 0;
+    for (const Inst& in : E->h_inst) {
+        int64_t strips = 1;
+        if (in.ny > 64 - H) strips += (in.ny - (64 - H) + (64 - 2 * H) - 1) / (64 - 2 * H);
+        n += strips * ((in.nx + rows - 1) / rows);
+    }
+    return n;
+}
+
+static int64_t simd_count(const Engine* E) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, E->device) != hipSuccess) return 1024;
+    return 4LL * prop.multiProcessorCount;
+}
+
 static shll_status engine_autotune(Engine* E, int32_t trial_steps, int64_t shape_class,
                                    shll_status (*advance)(Engine*, int64_t)) {
     const std::string key = tune_key(E, shape_class);
@@ -1255,11 +1279,22 @@ static shll_status engine_autotune(Engine* E, int32_t trial_steps, int64_t shape
         for (int k : {1, 2, 3, 4, 8})
             for (int w : {1, 4}) {
                 if (!step2d_supported(E->prec, E->order, k)) continue;
-                Launch L = start;
-                L.k = k;
-                L.wpb = w;
-                L.graph = std::max(64, 2 * k);
-                cands.push_back(L);
+                // shorter items when the default ones give fewer than 4 wavefronts per SIMD:
+                // each wavefront walks its rows one after another, so a launch of few items
+                // is latency-bound, and more items trade halo rows for parallelism
+                const int rd = rows2d(k);
+                std::vector<int> rows = {1};
+                if (items2d(E, k, rd) < 4 * simd_count(E))
+                    for (int r : {rd * 3 / 4, rd / 2, rd * 3 / 8, rd / 4, rd * 3 / 16, rd / 8})
+                        if (r >= 2) rows.push_back(r);
+                for (int r : rows) {
+                    Launch L = start;
+                    L.C = r;
+                    L.k = k;
+                    L.wpb = w;
+                    L.graph = std::max(64, 2 * k);
+                    cands.push_back(L);
+                }
             }
     }
     double best = 1e300;

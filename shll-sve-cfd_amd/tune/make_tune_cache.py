@@ -1,9 +1,9 @@
 #!/usr/bin/env python3
 """Run the library's autotuner (shll_autotune / shll_batch_autotune) on the benchmark
 problems and record its choices in a tuning cache (one line per device, kernel and
-shape class: "<key>|C k waves graph").  The library reads the cache named by
-$SHLL_TUNE_CACHE; bench.py points it at tune/gfx950.txt so timed runs use the
-recorded choice instead of re-tuning.
+shape class: "<key>|C k waves graph"; in 2D C is the rows per work item, 1 the default).
+The library reads the cache named by $SHLL_TUNE_CACHE; bench.py points it at
+tune/gfx950.txt so timed runs use the recorded choice instead of re-tuning.
 
 Usage (on the GPU):  python shll-sve-cfd_amd/tune/make_tune_cache.py OUT.txt [--only PREC ...]
   --only f64f c32k ...: tune only the problems of these precisions and append their lines to
@@ -71,6 +71,16 @@ def main():
         with S.Solver(S.problem(2, 1, 4096, 4096, precision=prec), S.ic_circle(4096, 4096)) as s:
             s.autotune(256)
             print("single dim2 order1 prec%d 4096^2:" % prec, s.get_launch(), flush=True)
+    # 2D second order: the reference OpenMP solver's problem (MC limiter, theta 1.25, four
+    # contacts, 1024^2; base-omp/2nd_order_base_shll.c), every precision; its default items give
+    # about one wavefront per SIMD, so the autotuner also tries shorter ones (1000-step trial)
+    for prec in (S.REF32, S.F64, S.C32, S.C32K):
+        if not wanted(prec):
+            continue
+        pb = S.problem(2, 2, 1024, 1024, precision=prec, limiter=S.LIM_MC, theta=1.25)
+        with S.Solver(pb, S.ic_four_contacts(1024, 1024)) as s:
+            s.autotune(1000)
+            print("single dim2 order2 MC prec%d 1024^2:" % prec, s.get_launch(), flush=True)
     print(open(out).read())
 
 

@@ -226,6 +226,42 @@ def test_2d_launch_shapes(orc):
         assert_same(pb, gpu_run(pb, ic, 12, launch), ora, "2d shape %s" % (launch,))
 
 
+@pytest.mark.parametrize("prec", [S.F64, S.REF32, S.C32K])
+def test_2d_rows_per_item(orc, prec):
+    """2D work items of any height (cells_per_lane in 2D: the owned rows of an item, 2 and
+    more; 0 and 1 the default) give the oracle's bits: second order (MC, both walls) at one
+    step per launch, the compensated form's two-step pipeline, first order at 4 steps, and a
+    batch; heights above the grid, and bad heights rejected."""
+    for order, lim, k, rows in ((2, S.LIM_MC, 1, (2, 3, 5, 7, 300)), (1, S.LIM_MINMOD, 4, (2, 3, 17, 64))):
+        for bc in (0, 1):
+            pb = S.problem(2, order, 150, 130, precision=prec, bc=bc, limiter=lim)
+            ic = icgen.random_2d(150, 130, 71 + bc, block=9)
+            ora = oracle_run(orc, pb, ic, 9)
+            for r in rows:
+                banner("2d rows", prec, order, bc, r)
+                assert_same(pb, gpu_run(pb, ic, 9, (r, k, 4 if r % 2 else 1, 0)), ora, "2d rows %d o%d" % (r, order))
+    if prec == S.C32K:
+        pb = S.problem(2, 2, 203, 70, precision=prec, bc=1)
+        ic = icgen.random_2d(203, 70, 77, block=9)
+        ora = oracle_run(orc, pb, ic, 7)
+        for r in (2, 5, 12):
+            assert_same(pb, gpu_run(pb, ic, 7, (r, 2, 1, 4)), ora, "2d o2 k2 rows %d" % r)
+    pbs, prims = S.domain_batch(59, 4, smin=20, smax=120, precision=prec, order=2)
+    b = S.Batch(pbs, prims, [11, 5, 9, 2])
+    b.set_launch(6, 1, 4, 4)
+    assert b.get_launch() == (6, 1, 4, 4)
+    b.run()
+    for k in range(4):
+        ref = oracle_run(orc, pbs[k], prims[k], [11, 5, 9, 2][k])
+        assert_oracle_match(prec, b.conserved(k), ref[1], "rows batch instance %d" % k)
+    b.close()
+    with S.Solver(S.problem(2, 2, 40, 30, precision=prec), S.ic_circle(40, 30)) as s:
+        for bad in (-1, 4097):
+            with pytest.raises(S.ShllError) as e:
+                s.set_launch(bad, 1, 1, 0)
+            assert e.value.code == S.EINVAL
+
+
 @pytest.mark.parametrize("prec", [S.F64, S.C32, S.C32K])
 @pytest.mark.parametrize("order", [1, 2])
 def test_1d_batch_equals_single_runs(orc, prec, order):

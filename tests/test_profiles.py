@@ -31,7 +31,7 @@ CASES = {
     ("domains", "f64"): ("dim2 order1 prec0 batch1 class18", "k_step2d_tb_f64_domains64"),
     ("omp2d", "ref32"): ("dim2 order2mc prec1 batch0 class20", "k_step2d_ref32_o2_mc1024"),
     ("omp2d", "f64"): ("dim2 order2mc prec0 batch0 class20", "k_step2d_f64_o2_mc1024"),
-    ("omp2d", "c32k"): ("dim2 order2mc prec3 batch0 class20", "k_step2d_tb2_c32k_o2_mc1024"),
+    ("omp2d", "c32k"): ("dim2 order2mc prec3 batch0 class20", "k_step2d_c32k_o2_mc1024"),
 }
 
 
