@@ -289,7 +289,11 @@ class Solver:
             _check(lib().shll_compare_host(self.h, _dp(ref), ctypes.byref(e)))
         return e.as_dict()
 
-    def set_launch(self, cells_per_lane=2, steps_per_launch=8, waves_per_block=4, graph_steps=128):
+    def set_launch(self, cells_per_lane=None, steps_per_launch=8, waves_per_block=4, graph_steps=128):
+        """cells_per_lane: 1D cells per lane (default 2); 2D owned rows per work item (default 1:
+        16, 32 or 64 rows for 1, 2 or more steps per launch)."""
+        if cells_per_lane is None:
+            cells_per_lane = 2 if self.pb.dim == 1 else 1
         _check(lib().shll_set_launch(self.h, cells_per_lane, steps_per_launch, waves_per_block, graph_steps))
 
     def get_launch(self):
@@ -365,7 +369,10 @@ class Batch:
         _check(lib().shll_batch_conserved_sums(self.h, _dp(out)))
         return out.reshape(-1, 4)
 
-    def set_launch(self, cells_per_lane=2, steps_per_launch=16, waves_per_block=4, graph_steps=128):
+    def set_launch(self, cells_per_lane=None, steps_per_launch=16, waves_per_block=4, graph_steps=128):
+        """As Solver.set_launch (1D default 2 cells per lane, 2D the default rows per item)."""
+        if cells_per_lane is None:
+            cells_per_lane = 2 if self.pbs[0].dim == 1 else 1
         _check(lib().shll_batch_set_launch(self.h, cells_per_lane, steps_per_launch, waves_per_block, graph_steps))
 
     def autotune(self, trial_steps):
