@@ -1279,13 +1279,15 @@ static shll_status engine_autotune(Engine* E, int32_t trial_steps, int64_t shape
         for (int k : {1, 2, 3, 4, 8})
             for (int w : {1, 4}) {
                 if (!step2d_supported(E->prec, E->order, k)) continue;
-                // shorter items when the default ones give fewer than 4 wavefronts per SIMD:
-                // each wavefront walks its rows one after another, so a launch of few items
-                // is latency-bound, and more items trade halo rows for parallelism
+                // items a little longer or shorter than the default change how many rounds
+                // of resident wavefronts a launch takes, whose last round may run part-empty;
+                // much shorter items when the default ones give fewer than 4 wavefronts per
+                // SIMD: each wavefront walks its rows one after another, so a launch of few
+                // items is latency-bound, and more items trade halo rows for parallelism
                 const int rd = rows2d(k);
-                std::vector<int> rows = {1};
+                std::vector<int> rows = {1, rd * 3 / 4, rd * 5 / 4, rd * 3 / 2};
                 if (items2d(E, k, rd) < 4 * simd_count(E))
-                    for (int r : {rd * 3 / 4, rd / 2, rd * 3 / 8, rd / 4, rd * 3 / 16, rd / 8})
+                    for (int r : {rd / 2, rd * 3 / 8, rd / 4, rd * 3 / 16, rd / 8})
                         if (r >= 2) rows.push_back(r);
                 for (int r : rows) {
                     Launch L = start;
