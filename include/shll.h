@@ -190,7 +190,9 @@ shll_status shll_get_launch(const shll_ctx *ctx, int32_t out[4]);
  * 1 or 4 waves per block (and 256-4096 steps per launch when the whole tube fits one
  * segment or block, which then has no halo); 2D the valid steps per launch, 1 or 4
  * waves per block, work items of 3/4, 5/4 and 3/2 of the default rows, and shorter ones
- * (down to 1/8) when the default ones give fewer than 4 wavefronts per SIMD. */
+ * (down to 1/8) when the default ones give fewer than 4 wavefronts per SIMD; graphs of
+ * 128 (2D: 64) steps, or two launches if longer, and for a single instance also of 2048
+ * (2D: 1024) steps and direct launches (graph_steps 0). */
 shll_status shll_autotune(shll_ctx *ctx, int32_t trial_steps);
 
 /* Average duration (ms) of the step kernel launches of the last shll_step /

@@ -443,36 +443,48 @@ static hipError_t launch_one(const Engine* E, const StepArgs& a, int64_t n_items
 
 // Single instance: every launch has the same items; ksteps varies only on the last one.
 // Graph of g launches (g even) captured per buffer parity and replayed.
+static int single_graph_launches(const Engine* E, int k) {
+    return E->L.graph > 0 ? std::max(2, (E->L.graph / k) & ~1) : 0;
+}
+
+// The graph of g launches starting from buffer parity p: captured and instantiated once per
+// launch shape (set_launch captures both parities, so a timed run never includes a capture)
+static shll_status single_graph(Engine* E, int p, int g) {
+    const int key[3] = {E->L.C, E->L.k, g};
+    if (E->gexec[p] && std::memcmp(key, E->g_key, sizeof(key)) != 0) engine_drop_graphs(E);
+    if (E->gexec[p]) return SHLL_OK;
+    const int64_t n_items = E->n_items;
+    hipGraph_t graph;
+    HIPCHECK(hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal));
+    for (int q = 0; q < g; ++q) {
+        StepArgs a = make_args(E, 0, p ^ (q & 1), n_items);
+        hipError_t e = launch_one(E, a, n_items);
+        if (e != hipSuccess) {
+            hipGraph_t junk;
+            (void)hipStreamEndCapture(E->stream, &junk);
+            return fail(SHLL_EDEVICE, "launch during capture: %s", hipGetErrorString(e));
+        }
+    }
+    HIPCHECK(hipStreamEndCapture(E->stream, &graph));
+    hipError_t e = hipGraphInstantiate(&E->gexec[p], graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(SHLL_EDEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    std::memcpy(E->g_key, key, sizeof(key));
+    return SHLL_OK;
+}
+
 static shll_status single_steps(Engine* E, int64_t nsteps) {
     const int k = launch_k(E);
     int64_t left = nsteps;
     const int64_t n_items = E->n_items;
     int64_t launches = 0;
-    const int g = E->L.graph > 0 ? std::max(2, (E->L.graph / k) & ~1) : 0;
+    const int g = single_graph_launches(E, k);
     if (E->timing) HIPCHECK(hipEventRecord(E->ev0, E->stream));
     while (left > 0) {
         if (g > 0 && left >= (int64_t)g * k) {
             const int p = E->cur;
-            const int key[3] = {E->L.C, E->L.k, g};
-            if (E->gexec[p] && std::memcmp(key, E->g_key, sizeof(key)) != 0) engine_drop_graphs(E);
-            if (!E->gexec[p]) {
-                hipGraph_t graph;
-                HIPCHECK(hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal));
-                for (int q = 0; q < g; ++q) {
-                    StepArgs a = make_args(E, 0, p ^ (q & 1), n_items);
-                    hipError_t e = launch_one(E, a, n_items);
-                    if (e != hipSuccess) {
-                        hipGraph_t junk;
-                        (void)hipStreamEndCapture(E->stream, &junk);
-                        return fail(SHLL_EDEVICE, "launch during capture: %s", hipGetErrorString(e));
-                    }
-                }
-                HIPCHECK(hipStreamEndCapture(E->stream, &graph));
-                hipError_t e = hipGraphInstantiate(&E->gexec[p], graph, nullptr, nullptr, 0);
-                (void)hipGraphDestroy(graph);
-                if (e != hipSuccess) return fail(SHLL_EDEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e));
-                std::memcpy(E->g_key, key, sizeof(key));
-            }
+            shll_status st = single_graph(E, p, g);
+            if (st != SHLL_OK) return st;
             HIPCHECK(hipGraphLaunch(E->gexec[p], E->stream));
             left -= (int64_t)g * k;
             launches += g;
@@ -1150,6 +1162,10 @@ static shll_status set_launch(Engine* E, int32_t C, int32_t k, int32_t wpb, int3
         (void)engine_build_items(E);
         return st;
     }
+    // a single instance's graphs depend on the launch shape only: capture them now
+    const int g = E->batch ? 0 : single_graph_launches(E, launch_k(E));
+    for (int p = 0; p < 2 && g > 0; ++p)
+        if ((st = single_graph(E, p, g)) != SHLL_OK) return st;
     return SHLL_OK;
 }
 
@@ -1269,10 +1285,21 @@ static shll_status engine_autotune(Engine* E, int32_t trial_steps, int64_t shape
                 for (int k : ks) {
                     // batches: fewer than 4 steps per launch only multiplies the launches
                     if (E->batch && k < 4) continue;
-                    Launch L = start;
-                    L.C = C; L.k = k; L.wpb = w;
-                    L.graph = std::max(128, 2 * k);
-                    cands.push_back(L);
+                    // single instances, whose launches are short, also try long graphs and
+                    // direct launches (kernels of ~10 us outlast the host's launch cost, and a
+                    // graph's nodes run with gaps between them: the 65536-cell tube ran 8%
+                    // faster launched directly than in 128-step graphs)
+                    std::vector<int> graphs = {std::max(128, 2 * k)};
+                    if (!E->batch) {
+                        if (graphs[0] < 2048) graphs.push_back(2048);
+                        graphs.push_back(0);
+                    }
+                    for (int g : graphs) {
+                        Launch L = start;
+                        L.C = C; L.k = k; L.wpb = w;
+                        L.graph = g;
+                        cands.push_back(L);
+                    }
                 }
             }
     } else {
@@ -1289,14 +1316,20 @@ static shll_status engine_autotune(Engine* E, int32_t trial_steps, int64_t shape
                 if (items2d(E, k, rd) < 4 * simd_count(E))
                     for (int r : {rd / 2, rd * 3 / 8, rd / 4, rd * 3 / 16, rd / 8})
                         if (r >= 2) rows.push_back(r);
-                for (int r : rows) {
-                    Launch L = start;
-                    L.C = r;
-                    L.k = k;
-                    L.wpb = w;
-                    L.graph = std::max(64, 2 * k);
-                    cands.push_back(L);
+                std::vector<int> graphs = {std::max(64, 2 * k)};
+                if (!E->batch) {
+                    graphs.push_back(1024);
+                    graphs.push_back(0);
                 }
+                for (int r : rows)
+                    for (int g : graphs) {
+                        Launch L = start;
+                        L.C = r;
+                        L.k = k;
+                        L.wpb = w;
+                        L.graph = g;
+                        cands.push_back(L);
+                    }
             }
     }
     double best = 1e300;
@@ -1308,8 +1341,10 @@ static shll_status engine_autotune(Engine* E, int32_t trial_steps, int64_t shape
         if (set_launch(E, L.C, L.k, L.wpb, L.graph) != SHLL_OK) continue;
         if ((st = engine_snapshot(E, false)) != SHLL_OK) break;
         engine_rewind(E);
-        // warm-up (also captures graphs), then the timed trial
-        if ((st = advance(E, std::max<int64_t>(1, trial_steps / 4))) != SHLL_OK) break;
+        // warm-up (also captures graphs: a single instance runs two chunks of the candidate's
+        // graph, so the timed trial never includes a capture), then the timed trial
+        const int64_t warm = std::max<int64_t>(std::max<int64_t>(1, trial_steps / 4), E->batch ? 0 : 2LL * L.graph);
+        if ((st = advance(E, warm)) != SHLL_OK) break;
         if ((st = engine_snapshot(E, false)) != SHLL_OK) break;
         engine_rewind(E);
         auto t0 = std::chrono::steady_clock::now();

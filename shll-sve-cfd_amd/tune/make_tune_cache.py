@@ -44,12 +44,13 @@ def main():
         b.close()
     # single instances: 1D first order 1024 cells, 1D second order 65536 cells
     # (a 65536-cell trial of 2000 steps lasts ~3 ms per candidate, short enough for run-to-run
-    # noise to pick among shapes within 5% of each other; 8000 steps)
+    # noise to pick among shapes within 5% of each other, and 8000-step trials ranked graph
+    # lengths unlike the 104858-step run; 30000 steps)
     for pb, trial in ((S.problem(1, 1, 1024, precision=S.F64), 400),
-                      (S.problem(1, 2, 65536, precision=S.F64), 8000),
-                      (S.problem(1, 2, 65536, precision=S.F64F), 8000),
-                      (S.problem(1, 2, 65536, precision=S.C32), 8000),
-                      (S.problem(1, 2, 65536, precision=S.C32K), 8000)):
+                      (S.problem(1, 2, 65536, precision=S.F64), 30000),
+                      (S.problem(1, 2, 65536, precision=S.F64F), 30000),
+                      (S.problem(1, 2, 65536, precision=S.C32), 30000),
+                      (S.problem(1, 2, 65536, precision=S.C32K), 30000)):
         if not wanted(pb.precision):
             continue
         with S.Solver(pb, S.ic_tube(pb.nx)) as s:
