@@ -58,6 +58,7 @@ def lib():
         L.orc_run.argtypes = [P, ctypes.c_int, dp, ctypes.c_int64, dp, dp, ctypes.c_int, dp]
         L.orc_run_c32_raw.argtypes = [P, ctypes.c_int, dp, ctypes.c_int64, fp, fp]
         L.orc_stage.argtypes = [P, ctypes.c_int, dp, dp, dp, dp, dp, dp, dp, dp]
+        L.orc_f64f_rsq_nudge.argtypes = [ctypes.c_int]
         L.orc_minmod.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double]
         L.orc_minmod.restype = ctypes.c_double
         L.orc_ic_tube.argtypes = [ctypes.c_int] + [ctypes.c_double] * 7 + [dp]
@@ -134,6 +135,11 @@ def stage(pb, mode, prim):
     if pb.dim == 1:
         del out["hp"], out["hm"]
     return out
+
+
+def f64f_rsq_nudge(ulps):
+    """Test switch: the F64F mode's 1/sqrt moved by a pseudo-random 0..ulps ulp (0: off)."""
+    lib().orc_f64f_rsq_nudge(int(ulps))
 
 
 def minmod(mode, l, r):

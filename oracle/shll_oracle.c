@@ -31,7 +31,9 @@
  *       conservation, wall impulse, exact Riemann convergence) and by its distance to
  *       the F64 mode, which is FP64 round-off only: elementwise relative 1e-13 after
  *       short runs of either order and L1(rho) <= 1e-10 at t = 0.2 on second-order
- *       tubes, where a wrong sign, index or dropped term shows as 1e-4 or more.
+ *       tubes, where a wrong sign, index or dropped term shows as 1e-4 or more.  Its
+ *       1/sqrt moved by 0-2 ulp (orc_f64f_rsq_nudge, the kernels' faithful sequence)
+ *       moves long runs by 2e-14-3e-14, the size of the kernels' distance to it.
  *   Initial conditions: bitwise against the reference's own initial arrays
  *       (golden step-0 dumps) for Sod, implosion and four shocks; the circle by a
  *       brute-force count of cells inside the radius.
@@ -468,6 +470,26 @@ static int f64f_alloc(f64fgrid *g, const orc_problem *pb, const double *prim)
     return 0;
 }
 
+static int f64f_rsq_ulps = 0; /* orc_f64f_rsq_nudge */
+
+void orc_f64f_rsq_nudge(int ulps) { f64f_rsq_ulps = ulps < 0 ? -ulps : ulps; }
+
+/* 1.0/sqrt(x), moved by the test switch's pseudo-random offset when it is set */
+static double f64f_rsq(double x)
+{
+    double y = 1.0 / sqrt(x);
+    uint64_t b;
+    int d, i;
+    if (!f64f_rsq_ulps) return y;
+    memcpy(&b, &x, sizeof(b));
+    b ^= b >> 29;
+    b *= 0xbf58476d1ce4e5b9ULL;
+    b ^= b >> 32;
+    d = (int)(b % (uint64_t)(2 * f64f_rsq_ulps + 1)) - f64f_rsq_ulps;
+    for (i = 0; i < (d < 0 ? -d : d); i++) y = nextafter(y, d > 0 ? INFINITY : 0.0);
+    return y;
+}
+
 /* split fluxes of one cell scaled by sc (base_shll.c:135-157 in the F64F
  * arrangement; the step passes DT/DX, the stage entry 1) and the sound speed */
 static void f64f_cell_flux(f64fgrid *g, int c, double sc)
@@ -478,7 +500,7 @@ static void f64f_cell_flux(f64fgrid *g, int c, double sc)
     double vx = m * inv;
     double P = fma(-0.5 * m, vx, E) * g->rcv; /* rho R T, R = 1 */
     double x = g->gR * (P * inv);            /* a^2 */
-    double y = 1.0 / sqrt(x);                /* 1/a */
+    double y = f64f_rsq(x);                  /* 1/a */
     double M = vx * y, hs = 0.5 * sc;
     double z1 = fma(hs, M, hs), z3 = fma(hs, M, -hs), z2 = ((hs * x) * y) * fma(-M, M, 1.0);
     double f[3], U[3];

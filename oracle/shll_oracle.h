@@ -109,6 +109,13 @@ int orc_stage(const orc_problem *pb, int mode, const double *prim_in, double *co
 
 /* minmod of the second-order file, in float (REF32) or double (F64). */
 double orc_minmod(int mode, double l, double r);
+
+/* Test switch of the F64F mode (process-wide; 0, the default, leaves it as restated):
+ * move its 1/sqrt(gamma R T) by a pseudo-random number of units in the last place between
+ * -ulps and +ulps, chosen by a hash of the argument (so runs stay deterministic).  It
+ * models the kernels' faithful reciprocal square root, which lies within 2 ulp of the
+ * correctly rounded value, to bound the differences that causes. */
+void orc_f64f_rsq_nudge(int ulps);
 /* Limited slope of an interior cell from the values q at c-1, c, c+1 (ORC_LIM_*). */
 double orc_slope(int mode, int limiter, double theta, double ql, double qc, double qr);
 
