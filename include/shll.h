@@ -179,7 +179,8 @@ void shll_free(shll_ctx *ctx);
  * 1, 2, 3, 4 or 8 in first order; 1, or 2 for SHLL_C32/SHLL_C32K in second order),
  * waves_per_block in {1,4} (1D, 4 with 2+ cells per lane: the block's wavefronts hold
  * one segment and share their edge fluxes), graph_steps: steps captured per HIP graph
- * (0 = no graph).  shll_get_launch returns the values as set (a 2D 0 reads back as 1).
+ * (0 = no graph; at most 4096 launches per graph; a single instance's graphs are captured
+ * when the shape is set).  shll_get_launch returns the values as set (a 2D 0 reads back as 1).
  * Results do not depend on the launch shape (bitwise). */
 shll_status shll_set_launch(shll_ctx *ctx, int32_t cells_per_lane, int32_t steps_per_launch,
                             int32_t waves_per_block, int32_t graph_steps);

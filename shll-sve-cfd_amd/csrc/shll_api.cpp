@@ -179,6 +179,7 @@ struct Launch {
 // column); 0 or 1 selects this default.
 static int rows2d(int k) { return k <= 1 ? 16 : (k == 2 ? 32 : 64); }
 static constexpr int kMaxRows2d = 4096;
+static constexpr int kMaxGraphLaunches = 4096;  // launches per captured graph
 
 // 1D tiling of one instance of nx cells: segments of XW*64*C cells (XW wavefronts of a
 // cooperative block, else 1), halo order*k.  The segment that holds a wall also holds
@@ -1143,6 +1144,11 @@ static shll_status set_launch(Engine* E, int32_t C, int32_t k, int32_t wpb, int3
     if (E->dim == 2 && (C < 0 || C > kMaxRows2d))
         return fail(SHLL_EINVAL, "2D rows per item (cells_per_lane) must be 0-%d (got %d)", kMaxRows2d, C);
     if (k < 1 || graph < 0) return fail(SHLL_EINVAL, "steps per launch must be >= 1, graph steps >= 0");
+    // a graph is captured when the shape is set (single instances) or first replayed: bound
+    // its size
+    if (graph / k > kMaxGraphLaunches)
+        return fail(SHLL_EINVAL, "graph steps / steps per launch must be at most %d (got %d / %d)", kMaxGraphLaunches,
+                    graph, k);
     if (E->dim == 2 && !step2d_supported(E->prec, E->order, k))
         return fail(SHLL_EINVAL, "2D steps per launch: 1, 2, 3, 4 or 8 in first order; 1, or 2 in the "
                                  "FP32-compensated second order (got %d)", k);

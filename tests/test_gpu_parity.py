@@ -155,6 +155,9 @@ def test_1d_impossible_launch_shape_is_rejected():
         with pytest.raises(S.ShllError) as e:
             s.set_launch(2, 64, 1, 0)  # 128-cell segments, 128-cell halo each side
         assert e.value.code == S.EINVAL
+        with pytest.raises(S.ShllError) as e:
+            s.set_launch(2, 1, 1, 4097)  # a graph of more than 4096 launches
+        assert e.value.code == S.EINVAL
         assert s.get_launch() == before
         s.step(3)
 
