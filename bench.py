@@ -928,7 +928,10 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true", help="skip the FP64 referee of the batch")
     ap.add_argument("--progress", type=int, default=0,
                     help="advance the timed run in pieces of this many steps and log each (profiling runs)")
-    ap.add_argument("--launch", default="", help="C,k,waves,graph (default: the recorded autotuner choice)")
+    ap.add_argument("--launch", default="",
+                    help="C,k,waves,graph: C cells per lane (1D) or rows per work item (2D), k steps per launch, "
+                         "waves per block, steps per graph (0: direct launches); default: the recorded autotuner "
+                         "choice")
     ap.add_argument("--calib-copy", type=int, default=0, help="bytes of a device copy to run first (profiling)")
     ap.add_argument("--autotune", action="store_true", help="re-run the autotuner instead of its recorded choices")
     ap.add_argument("--form", default="f64f", choices=sorted(FORMS),
